@@ -1,0 +1,725 @@
+// C ABI of liboagpu.so (include/oagpu.h): context, uploads, pattern
+// grouping and the chunked scan driver.  The loop structure follows the
+// reference's partialNEQ_Blocked_STL_MD (src/Algorithm.cpp:322-886): a
+// trait block is prepared once (src/Algorithm.cpp:550-592), then SNP chunks
+// stream through prep -> contraction -> Gram -> corrections -> pair solve
+// -> statistics (src/Algorithm.cpp:598-827).  The reference's SNP blocks of
+// 500 become chunks sized to fill the GPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "oagpu_kernels.h"
+
+using namespace oa;
+
+struct oagpu_ctx {
+  oagpu_params prm{};
+  hipStream_t st = nullptr;
+  std::string err;
+  int n = 0, n_pad = 0, l = 0, r = 0, p = 0, Q = 0, lb = 0;
+  size_t op_size = 8, c_size = 8;
+
+  // covariates
+  bool have_cov = false;
+  std::vector<double> Stl_inv_diag;
+  double *XL_cm = nullptr, *XL_rm = nullptr, *U0 = nullptr, *Stl_inv_d = nullptr, *Stl_pk = nullptr;
+  void *XLop_hi = nullptr, *XLop_lo = nullptr;   // covariate rows of B, in the operand format
+
+  // interactions
+  double* E = nullptr;
+  long long lde = 0;
+  int n_env = 0;
+
+  // traits
+  bool have_traits = false;
+  int t = 0, ldt = 0, trait_offset = 0;
+  const void* Y = nullptr;
+  int ydt = 0;
+  long long ldy = 0;
+  void* Y_own = nullptr;
+  size_t Y_own_bytes = 0;
+  void *Bhi = nullptr, *Blo = nullptr;
+  int* rows = nullptr;
+  int *ymiss = nullptr, *yzero = nullptr, *ypat = nullptr;
+  double *yy = nullptr, *sst = nullptr, *rss0 = nullptr, *rt = nullptr, *zt = nullptr;
+  unsigned long long* hash = nullptr;
+  int* scratch_i = nullptr;   // rep_of / mismatch
+  int G = 0, max_pat_cnt = 0;
+  int *pat_cnt = nullptr, *pat_rep = nullptr, *pat_pd = nullptr;
+  double *UA = nullptr, *UA_invd = nullptr;
+  size_t pat_cap = 0;
+
+  // genotypes
+  bool have_geno = false;
+  const void* Gm = nullptr;
+  int gdt = 0;
+  long long ldg = 0;
+  int n_src = 0, m = 0, snp_offset = 0;
+  void* G_own = nullptr;
+  size_t G_own_bytes = 0;
+  int *col_src = nullptr, *col_env = nullptr;
+  size_t desc_cap = 0;
+
+  // chunk buffers
+  int mc_cap = 0;
+  void *Ahi = nullptr, *Alo = nullptr, *Cbuf = nullptr;
+  double *col_mean = nullptr, *Sbr = nullptr, *Str = nullptr, *varx = nullptr, *Ccorr = nullptr;
+  size_t Ahi_cap = 0, Alo_cap = 0, C_cap = 0, Ccorr_cap = 0;
+
+  // outputs
+  oagpu_hit* hits = nullptr;
+  long long hit_cap = 0;
+  unsigned long long* counters = nullptr;
+  FallbackEntry* fb = nullptr;
+  int fb_cap = 0;
+  double timings[8] = {0};
+};
+
+#define HIPCK(ctx, expr)                                                              \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess) {                                                           \
+      (ctx)->err = std::string("HIP error: ") + hipGetErrorString(e_) + " at " #expr; \
+      return OAGPU_ERR_HIP;                                                           \
+    }                                                                                 \
+  } while (0)
+
+static int fail(oagpu_ctx* c, int code, const std::string& msg) {
+  c->err = msg;
+  return code;
+}
+
+// (Re)allocate a device buffer so it holds at least `bytes`.
+static hipError_t grow(void** p, size_t* cap, size_t bytes) {
+  if (*p && *cap >= bytes) return hipSuccess;
+  if (*p) {
+    hipError_t e = hipFree(*p);
+    if (e != hipSuccess) return e;
+    *p = nullptr;
+  }
+  *cap = 0;
+  hipError_t e = hipMalloc(p, bytes > 0 ? bytes : 64);
+  if (e == hipSuccess) *cap = bytes;
+  return e;
+}
+
+static void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+static double erfinv_host(double x) {
+  if (x <= -1.0) return -INFINITY;
+  if (x >= 1.0) return INFINITY;
+  // Giles (2010) single-precision start, refined by Newton steps on erf.
+  double w = -std::log((1.0 - x) * (1.0 + x)), y;
+  if (w < 5.0) {
+    w -= 2.5;
+    double q = 2.81022636e-08;
+    q = 3.43273939e-07 + q * w; q = -3.5233877e-06 + q * w;
+    q = -4.39150654e-06 + q * w; q = 0.00021858087 + q * w;
+    q = -0.00125372503 + q * w; q = -0.00417768164 + q * w;
+    q = 0.246640727 + q * w; q = 1.50140941 + q * w;
+    y = q * x;
+  } else {
+    w = std::sqrt(w) - 3.0;
+    double q = -0.000200214257;
+    q = 0.000100950558 + q * w; q = 0.00134934322 + q * w;
+    q = -0.00367342844 + q * w; q = 0.00573950773 + q * w;
+    q = -0.0076224613 + q * w; q = 0.00943887047 + q * w;
+    q = 1.00167406 + q * w; q = 2.83297682 + q * w;
+    y = q * x;
+  }
+  for (int it = 0; it < 4; ++it) {
+    const double e = std::erf(y) - x;
+    y -= e / (1.1283791670955126 * std::exp(-y * y));
+  }
+  return y;
+}
+
+extern "C" double oagpu_tvalue(float pval) {
+  return (double)(float)(erfinv_host(1.0 - (double)pval) * std::sqrt(2.0));
+}
+
+// Upper Cholesky of a packed symmetric matrix (LAPACK potrf('U') rule).
+static bool chol_packed(std::vector<double>& A, int l) {
+  for (int j = 0; j < l; ++j) {
+    double ajj = A[upk(j, j)];
+    for (int k = 0; k < j; ++k) ajj -= A[upk(k, j)] * A[upk(k, j)];
+    if (!(ajj > 0.0)) return false;
+    ajj = std::sqrt(ajj);
+    A[upk(j, j)] = ajj;
+    for (int i = j + 1; i < l; ++i) {
+      double aji = A[upk(j, i)];
+      for (int k = 0; k < j; ++k) aji -= A[upk(k, j)] * A[upk(k, i)];
+      A[upk(j, i)] = aji / ajj;
+    }
+  }
+  return true;
+}
+
+// Smallest singular value of XL via Jacobi on XL'XL (the reference uses
+// sgesvd for the same test, src/Algorithm.cpp:508-522).
+static double min_singular(const std::vector<double>& S, int l) {
+  std::vector<double> A = S;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0;
+    for (int i = 0; i < l; ++i)
+      for (int j = i + 1; j < l; ++j) off += A[i * l + j] * A[i * l + j];
+    if (off < 1e-300) break;
+    for (int p = 0; p < l; ++p)
+      for (int q = p + 1; q < l; ++q) {
+        const double apq = A[p * l + q];
+        if (apq == 0.0) continue;
+        const double th = (A[q * l + q] - A[p * l + p]) / (2.0 * apq);
+        const double tt = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+        const double c = 1.0 / std::sqrt(tt * tt + 1.0), s = tt * c;
+        for (int k = 0; k < l; ++k) {
+          const double akp = A[k * l + p], akq = A[k * l + q];
+          A[k * l + p] = c * akp - s * akq;
+          A[k * l + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < l; ++k) {
+          const double apk = A[p * l + k], aqk = A[q * l + k];
+          A[p * l + k] = c * apk - s * aqk;
+          A[q * l + k] = s * apk + c * aqk;
+        }
+      }
+  }
+  double mn = INFINITY;
+  for (int i = 0; i < l; ++i) {
+    const double ev = A[i * l + i];
+    mn = std::min(mn, ev > 0 ? std::sqrt(ev) : 0.0);
+  }
+  return mn;
+}
+
+static DevScan make_dev(const oagpu_ctx* c) {
+  DevScan d{};
+  d.n = c->n; d.n_pad = c->n_pad; d.l = c->l; d.r = c->r; d.p = c->p; d.Q = c->Q;
+  d.t = c->t; d.ldt = c->ldt; d.ldc = c->t + c->l; d.G = c->G;
+  d.start_h = c->prm.disp_cov ? 1 : c->l;
+  d.disp_cov = c->prm.disp_cov; d.store_all = c->prm.store_all; d.need_bl = c->prm.disp_cov;
+  d.min_r2_store = (double)c->prm.min_r2_store;
+  d.t_star = oagpu_tvalue(c->prm.min_p_store);
+  d.min_p_disp = (double)c->prm.min_p_disp;
+  d.XL_cm = c->XL_cm; d.XL_rm = c->XL_rm; d.U0 = c->U0; d.Stl_inv_diag = c->Stl_inv_d;
+  d.E = c->E; d.lde = c->lde;
+  d.Y = c->Y; d.ydt = c->ydt; d.ldy = c->ldy;
+  d.ymiss = c->ymiss; d.yzero = c->yzero; d.yy = c->yy; d.sst = c->sst; d.rss0 = c->rss0;
+  d.ypat = c->ypat; d.rt = c->rt; d.zt = c->zt;
+  d.pat_cnt = c->pat_cnt; d.pat_rep = c->pat_rep; d.rows = c->rows;
+  d.UA = c->UA; d.UA_invd = c->UA_invd; d.pat_pd = c->pat_pd;
+  d.Gm = c->Gm; d.gdt = c->gdt; d.ldg = c->ldg; d.col_src = c->col_src; d.col_env = c->col_env;
+  d.col_mean = c->col_mean; d.Sbr = c->Sbr; d.Str = c->Str; d.varx = c->varx; d.Ccorr = c->Ccorr;
+  d.Cbuf = c->Cbuf;
+  d.hits = c->hits; d.hit_cap = c->hit_cap; d.counters = c->counters;
+  d.trait_offset = c->trait_offset; d.snp_offset = c->snp_offset; d.m_total = c->m;
+  d.fb = c->fb; d.fb_cap = c->fb_cap;
+  return d;
+}
+
+extern "C" int oagpu_create(const oagpu_params* prm, oagpu_ctx** out) {
+  if (!prm || !out) return OAGPU_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return OAGPU_ERR_NO_DEVICE;
+  auto* c = new oagpu_ctx();
+  c->prm = *prm;
+  *out = c;
+  const int n = prm->n, l = prm->l, r = prm->r;
+  if (n <= 0 || l <= 0 || r <= 0 || prm->max_traits <= 0)
+    return fail(c, OAGPU_ERR_ARG, "n, l, r and max_traits must be positive");
+  if (l + r >= n)
+    return fail(c, OAGPU_ERR_ARG,
+                "Error! The amount of unknows (individuals|elements) must be greater than total of "
+                "variables (#covariates+#ind.variables)!");
+  if (r > 4 || (r == 1 && l > 32) || (r == 2 && l > 16) || (r >= 3 && l > 8))
+    return fail(c, OAGPU_ERR_UNSUPPORTED, "unsupported (l, r): compiled for r<=4 with l<=32/16/8");
+  if (prm->precision < 0 || prm->precision > 2) return fail(c, OAGPU_ERR_ARG, "bad precision");
+  if (prm->device < 0 || prm->device >= ndev) return fail(c, OAGPU_ERR_ARG, "bad device");
+  HIPCK(c, hipSetDevice(prm->device));
+  HIPCK(c, hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+  c->n = n; c->l = l; c->r = r; c->p = l + r;
+  c->Q = l * r + r * (r + 1) / 2;
+  c->n_pad = (n + 63) / 64 * 64;
+  c->lb = l <= 8 ? 8 : (l <= 16 ? 16 : 32);
+  c->op_size = prm->precision == OAGPU_FP64 ? 8 : (prm->precision == OAGPU_FP32 ? 4 : 2);
+  c->c_size = prm->precision == OAGPU_FP64 ? 8 : 4;
+  c->hit_cap = prm->max_hits > 0 ? prm->max_hits : (1ll << 22);
+  c->fb_cap = 1 << 16;
+  HIPCK(c, hipMalloc(&c->hits, sizeof(oagpu_hit) * (size_t)c->hit_cap));
+  HIPCK(c, hipMalloc(&c->counters, sizeof(unsigned long long) * CNT_N));
+  HIPCK(c, hipMalloc(&c->fb, sizeof(FallbackEntry) * (size_t)c->fb_cap));
+  // Trait-side buffers sized by max_traits.
+  const int T = prm->max_traits;
+  c->ldt = (T + 63) / 64 * 64;
+  const size_t brows = (size_t)((T + l + 127) / 128 * 128);
+  HIPCK(c, hipMalloc(&c->Bhi, brows * c->n_pad * c->op_size));
+  HIPCK(c, hipMemset(c->Bhi, 0, brows * c->n_pad * c->op_size));
+  if (prm->precision == OAGPU_BF16X3) {
+    HIPCK(c, hipMalloc(&c->Blo, brows * c->n_pad * c->op_size));
+    HIPCK(c, hipMemset(c->Blo, 0, brows * c->n_pad * c->op_size));
+  }
+  HIPCK(c, hipMalloc(&c->rows, sizeof(int) * (size_t)T * n));
+  HIPCK(c, hipMalloc(&c->ymiss, sizeof(int) * T));
+  HIPCK(c, hipMalloc(&c->yzero, sizeof(int) * T));
+  HIPCK(c, hipMalloc(&c->ypat, sizeof(int) * T));
+  HIPCK(c, hipMalloc(&c->scratch_i, sizeof(int) * T));
+  HIPCK(c, hipMalloc(&c->yy, sizeof(double) * T));
+  HIPCK(c, hipMalloc(&c->sst, sizeof(double) * T));
+  HIPCK(c, hipMalloc(&c->rss0, sizeof(double) * T));
+  HIPCK(c, hipMalloc(&c->rt, sizeof(double) * (size_t)l * c->ldt));
+  HIPCK(c, hipMalloc(&c->zt, sizeof(double) * (size_t)l * c->ldt));
+  HIPCK(c, hipMalloc(&c->hash, sizeof(unsigned long long) * 2 * T));
+  return OAGPU_OK;
+}
+
+extern "C" void oagpu_destroy(oagpu_ctx* c) {
+  if (!c) return;
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  void* bufs[] = {c->XL_cm, c->XL_rm, c->U0, c->Stl_inv_d, c->Stl_pk, c->XLop_hi, c->XLop_lo, c->E,
+                  c->Y_own, c->Bhi, c->Blo, c->rows, c->ymiss, c->yzero, c->ypat, c->yy, c->sst,
+                  c->rss0, c->rt, c->zt, c->hash, c->scratch_i, c->pat_cnt, c->pat_rep, c->pat_pd,
+                  c->UA, c->UA_invd, c->G_own, c->col_src, c->col_env, c->Ahi, c->Alo, c->Cbuf,
+                  c->col_mean, c->Sbr, c->Str, c->varx, c->Ccorr, c->hits, c->counters, c->fb};
+  for (void* b : bufs) dfree(b);
+  if (c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+}
+
+extern "C" const char* oagpu_last_error(const oagpu_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+extern "C" int oagpu_set_covariates(oagpu_ctx* c, const double* XL, long long ld) {
+  if (!c || !XL || ld < c->n) return c ? fail(c, OAGPU_ERR_ARG, "bad covariates") : OAGPU_ERR_ARG;
+  HIPCK(c, hipSetDevice(c->prm.device));
+  const int n = c->n, l = c->l, np = c->n_pad;
+  std::vector<double> S(l * l);
+  for (int i = 0; i < l; ++i)
+    for (int j = 0; j < l; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < n; ++k) s += XL[(size_t)i * ld + k] * XL[(size_t)j * ld + k];
+      S[i * l + j] = s;
+    }
+  for (int j = 0; j < l; ++j)
+    for (int k = 0; k < n; ++k)
+      if (std::isnan(XL[(size_t)j * ld + k]))
+        return fail(c, OAGPU_ERR_ARG, "covariates contain NaN (rows must be excluded first)");
+  if (min_singular(S, l) < 0.000001)
+    return fail(c, OAGPU_ERR_COVARIATES,
+                "Error, the covariate matrix contains linearly dependent columns. Please review and "
+                "change it accordingly.");
+  std::vector<double> pk(l * (l + 1) / 2);
+  for (int j = 0; j < l; ++j)
+    for (int i = 0; i <= j; ++i) pk[upk(i, j)] = S[i * l + j];
+  std::vector<double> U = pk;
+  if (!chol_packed(U, l)) return fail(c, OAGPU_ERR_COVARIATES, "covariate Gram matrix not positive definite");
+  // diag(Stl^-1) from the inverse of U.
+  std::vector<double> Ui(l * l, 0.0);
+  for (int j = 0; j < l; ++j) {
+    Ui[j * l + j] = 1.0 / U[upk(j, j)];
+    for (int i = j - 1; i >= 0; --i) {
+      double s = 0.0;
+      for (int k = i + 1; k <= j; ++k) s += U[upk(i, k)] * Ui[j * l + k];
+      Ui[j * l + i] = -s / U[upk(i, i)];
+    }
+  }
+  c->Stl_inv_diag.assign(l, 0.0);
+  for (int i = 0; i < l; ++i) {
+    double s = 0.0;
+    for (int j = i; j < l; ++j) s += Ui[j * l + i] * Ui[j * l + i];
+    c->Stl_inv_diag[i] = s;
+  }
+  std::vector<double> cm((size_t)l * np, 0.0), rm((size_t)n * l);
+  for (int j = 0; j < l; ++j)
+    for (int k = 0; k < n; ++k) {
+      cm[(size_t)j * np + k] = XL[(size_t)j * ld + k];
+      rm[(size_t)k * l + j] = XL[(size_t)j * ld + k];
+    }
+  if (!c->XL_cm) {
+    HIPCK(c, hipMalloc(&c->XL_cm, sizeof(double) * cm.size()));
+    HIPCK(c, hipMalloc(&c->XL_rm, sizeof(double) * rm.size()));
+    HIPCK(c, hipMalloc(&c->U0, sizeof(double) * pk.size()));
+    HIPCK(c, hipMalloc(&c->Stl_pk, sizeof(double) * pk.size()));
+    HIPCK(c, hipMalloc(&c->Stl_inv_d, sizeof(double) * l));
+    HIPCK(c, hipMalloc(&c->XLop_hi, c->op_size * (size_t)l * np));
+    if (c->prm.precision == OAGPU_BF16X3) HIPCK(c, hipMalloc(&c->XLop_lo, c->op_size * (size_t)l * np));
+  }
+  HIPCK(c, hipMemcpy(c->XL_cm, cm.data(), sizeof(double) * cm.size(), hipMemcpyHostToDevice));
+  HIPCK(c, hipMemcpy(c->XL_rm, rm.data(), sizeof(double) * rm.size(), hipMemcpyHostToDevice));
+  HIPCK(c, hipMemcpy(c->U0, U.data(), sizeof(double) * U.size(), hipMemcpyHostToDevice));
+  HIPCK(c, hipMemcpy(c->Stl_pk, pk.data(), sizeof(double) * pk.size(), hipMemcpyHostToDevice));
+  HIPCK(c, hipMemcpy(c->Stl_inv_d, c->Stl_inv_diag.data(), sizeof(double) * l, hipMemcpyHostToDevice));
+  // Covariate rows of the contraction operand B.
+  if (c->prm.precision == OAGPU_FP64) {
+    HIPCK(c, hipMemcpy(c->XLop_hi, cm.data(), sizeof(double) * cm.size(), hipMemcpyHostToDevice));
+  } else if (c->prm.precision == OAGPU_FP32) {
+    std::vector<float> f(cm.size());
+    for (size_t i = 0; i < cm.size(); ++i) f[i] = (float)cm[i];
+    HIPCK(c, hipMemcpy(c->XLop_hi, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice));
+  } else {
+    std::vector<uint16_t> hi(cm.size()), lo(cm.size());
+    for (size_t i = 0; i < cm.size(); ++i) {
+      const float x = (float)cm[i];
+      uint32_t u;
+      std::memcpy(&u, &x, 4);
+      u += 0x7FFFu + ((u >> 16) & 1u);
+      hi[i] = (uint16_t)(u >> 16);
+      uint32_t hb = ((uint32_t)hi[i]) << 16;
+      float hf;
+      std::memcpy(&hf, &hb, 4);
+      const float rest = x - hf;
+      std::memcpy(&u, &rest, 4);
+      u += 0x7FFFu + ((u >> 16) & 1u);
+      lo[i] = (uint16_t)(u >> 16);
+    }
+    HIPCK(c, hipMemcpy(c->XLop_hi, hi.data(), 2 * hi.size(), hipMemcpyHostToDevice));
+    HIPCK(c, hipMemcpy(c->XLop_lo, lo.data(), 2 * lo.size(), hipMemcpyHostToDevice));
+  }
+  c->have_cov = true;
+  c->have_traits = false;   // trait statistics depend on XL
+  return OAGPU_OK;
+}
+
+extern "C" int oagpu_set_interactions(oagpu_ctx* c, const double* E, long long ld, int n_env) {
+  if (!c || !E || ld < c->n || n_env <= 0) return c ? fail(c, OAGPU_ERR_ARG, "bad interactions") : OAGPU_ERR_ARG;
+  HIPCK(c, hipSetDevice(c->prm.device));
+  dfree(c->E);
+  c->E = nullptr;
+  std::vector<double> e((size_t)n_env * c->n);
+  for (int j = 0; j < n_env; ++j)
+    for (int k = 0; k < c->n; ++k) {
+      e[(size_t)j * c->n + k] = E[(size_t)j * ld + k];
+      if (std::isnan(e[(size_t)j * c->n + k]))
+        return fail(c, OAGPU_ERR_ARG, "interaction covariates contain NaN (rows must be excluded first)");
+    }
+  HIPCK(c, hipMalloc(&c->E, sizeof(double) * e.size()));
+  HIPCK(c, hipMemcpy(c->E, e.data(), sizeof(double) * e.size(), hipMemcpyHostToDevice));
+  c->lde = c->n;
+  c->n_env = n_env;
+  return OAGPU_OK;
+}
+
+static size_t dt_size(int dt) { return dt == OAGPU_F64 ? 8 : (dt == OAGPU_F32 ? 4 : 2); }
+
+extern "C" int oagpu_load_traits(oagpu_ctx* c, const void* Y, int dtype, int where, long long ld,
+                                 int t, int trait_offset) {
+  if (!c) return OAGPU_ERR_ARG;
+  if (!c->have_cov) return fail(c, OAGPU_ERR_STATE, "oagpu_set_covariates must come first");
+  if (!Y || t <= 0 || t > c->prm.max_traits || ld < c->n || dtype < 0 || dtype > 2 || dtype == OAGPU_F16)
+    return fail(c, OAGPU_ERR_ARG, "bad trait block (t <= max_traits, ld >= n, dtype F64/F32)");
+  HIPCK(c, hipSetDevice(c->prm.device));
+  const int n = c->n, l = c->l;
+  if ((long long)t * n > 0x7fffffffll) return fail(c, OAGPU_ERR_ARG, "t*n too large");
+  c->t = t;
+  c->trait_offset = trait_offset;
+  c->ydt = dtype;
+  c->ldy = ld;
+  if (where == OAGPU_HOST) {
+    const size_t bytes = dt_size(dtype) * ((size_t)ld * (t - 1) + n);
+    HIPCK(c, grow(&c->Y_own, &c->Y_own_bytes, bytes));
+    HIPCK(c, hipMemcpyAsync(c->Y_own, Y, bytes, hipMemcpyHostToDevice, c->st));
+    c->Y = c->Y_own;
+  } else {
+    c->Y = Y;
+  }
+  c->G = 0;
+  DevScan d = make_dev(c);
+  launch_ystats(d, c->prm.precision, c->Bhi, c->Blo, c->rows, c->ymiss, c->yzero, c->yy, c->sst, c->rt,
+                c->hash, c->st);
+  HIPCK(c, hipGetLastError());
+  // covariate rows of B after the traits
+  const size_t rowb = (size_t)c->n_pad * c->op_size;
+  HIPCK(c, hipMemcpyAsync((char*)c->Bhi + rowb * t, c->XLop_hi, rowb * l, hipMemcpyDeviceToDevice, c->st));
+  if (c->Blo)
+    HIPCK(c, hipMemcpyAsync((char*)c->Blo + rowb * t, c->XLop_lo, rowb * l, hipMemcpyDeviceToDevice, c->st));
+
+  // Group traits by missing-row pattern: key = (count, 128-bit hash),
+  // verified row by row on the device.
+  std::vector<int> ymiss(t);
+  std::vector<unsigned long long> hash(2 * (size_t)t);
+  HIPCK(c, hipMemcpyAsync(ymiss.data(), c->ymiss, sizeof(int) * t, hipMemcpyDeviceToHost, c->st));
+  HIPCK(c, hipMemcpyAsync(hash.data(), c->hash, sizeof(unsigned long long) * 2 * t, hipMemcpyDeviceToHost, c->st));
+  HIPCK(c, hipStreamSynchronize(c->st));
+  std::map<std::tuple<int, unsigned long long, unsigned long long>, int> key2rep;
+  std::vector<int> rep_of(t);
+  for (int y = 0; y < t; ++y) {
+    auto key = std::make_tuple(ymiss[y], hash[2 * y], hash[2 * y + 1]);
+    auto it = key2rep.find(key);
+    if (it == key2rep.end()) {
+      key2rep.emplace(key, y);
+      rep_of[y] = y;
+    } else {
+      rep_of[y] = it->second;
+    }
+  }
+  HIPCK(c, hipMemcpyAsync(c->scratch_i, rep_of.data(), sizeof(int) * t, hipMemcpyHostToDevice, c->st));
+  int* mism_d = c->ypat;   // ypat is rewritten below
+  launch_verify_patterns(t, n, c->rows, c->scratch_i, c->ymiss, mism_d, c->st);
+  HIPCK(c, hipGetLastError());
+  std::vector<int> mism(t);
+  HIPCK(c, hipMemcpyAsync(mism.data(), mism_d, sizeof(int) * t, hipMemcpyDeviceToHost, c->st));
+  HIPCK(c, hipStreamSynchronize(c->st));
+  std::vector<int> pat_of_rep(t, -1), ypat(t), pat_rep, pat_cnt;
+  for (int y = 0; y < t; ++y) {
+    const int rep = mism[y] ? y : rep_of[y];
+    if (pat_of_rep[rep] < 0) {
+      pat_of_rep[rep] = (int)pat_rep.size();
+      pat_rep.push_back(rep);
+      pat_cnt.push_back(ymiss[rep]);
+    }
+    ypat[y] = pat_of_rep[rep];
+  }
+  const int G = (int)pat_rep.size();
+  c->G = G;
+  c->max_pat_cnt = 0;
+  for (int g = 0; g < G; ++g) c->max_pat_cnt = std::max(c->max_pat_cnt, pat_cnt[g]);
+  if ((size_t)G > c->pat_cap) {
+    dfree(c->pat_cnt); dfree(c->pat_rep); dfree(c->pat_pd); dfree(c->UA); dfree(c->UA_invd);
+    c->pat_cap = (size_t)G;
+    HIPCK(c, hipMalloc(&c->pat_cnt, sizeof(int) * G));
+    HIPCK(c, hipMalloc(&c->pat_rep, sizeof(int) * G));
+    HIPCK(c, hipMalloc(&c->pat_pd, sizeof(int) * G));
+    HIPCK(c, hipMalloc(&c->UA, sizeof(double) * (size_t)G * (l * (l + 1) / 2)));
+    HIPCK(c, hipMalloc(&c->UA_invd, sizeof(double) * (size_t)G * l));
+  }
+  HIPCK(c, hipMemcpyAsync(c->pat_cnt, pat_cnt.data(), sizeof(int) * G, hipMemcpyHostToDevice, c->st));
+  HIPCK(c, hipMemcpyAsync(c->pat_rep, pat_rep.data(), sizeof(int) * G, hipMemcpyHostToDevice, c->st));
+  HIPCK(c, hipMemcpyAsync(c->ypat, ypat.data(), sizeof(int) * t, hipMemcpyHostToDevice, c->st));
+  d = make_dev(c);
+  launch_pattern(d, c->Stl_pk, G, c->pat_rep, c->UA, c->UA_invd, c->pat_pd, c->st);
+  HIPCK(c, hipGetLastError());
+  d = make_dev(c);
+  launch_trait_solve(d, c->lb, c->zt, c->rss0, c->st);
+  HIPCK(c, hipGetLastError());
+  HIPCK(c, hipStreamSynchronize(c->st));
+  c->have_traits = true;
+  return OAGPU_OK;
+}
+
+extern "C" int oagpu_load_genotypes(oagpu_ctx* c, const void* G, int dtype, int where, long long ld,
+                                    int n_src, int m, const int32_t* col_src, const int32_t* col_env,
+                                    int snp_offset) {
+  if (!c) return OAGPU_ERR_ARG;
+  if (!G || m <= 0 || n_src <= 0 || ld < c->n || dtype < 0 || dtype > 2)
+    return fail(c, OAGPU_ERR_ARG, "bad genotype block");
+  HIPCK(c, hipSetDevice(c->prm.device));
+  const int r = c->r;
+  if (!col_src && (long long)m * r > n_src) return fail(c, OAGPU_ERR_ARG, "m*r exceeds the source columns");
+  if (col_src)
+    for (long long i = 0; i < (long long)m * r; ++i)
+      if (col_src[i] < 0 || col_src[i] >= n_src) return fail(c, OAGPU_ERR_ARG, "column descriptor out of range");
+  if (col_env) {
+    for (long long i = 0; i < (long long)m * r; ++i)
+      if (col_env[i] >= c->n_env) return fail(c, OAGPU_ERR_ARG, "interaction column out of range");
+  }
+  c->gdt = dtype;
+  c->ldg = ld;
+  c->n_src = n_src;
+  c->m = m;
+  c->snp_offset = snp_offset;
+  if (where == OAGPU_HOST) {
+    const size_t bytes = dt_size(dtype) * ((size_t)ld * (n_src - 1) + c->n);
+    HIPCK(c, grow(&c->G_own, &c->G_own_bytes, bytes));
+    HIPCK(c, hipMemcpyAsync(c->G_own, G, bytes, hipMemcpyHostToDevice, c->st));
+    c->Gm = c->G_own;
+  } else {
+    c->Gm = G;
+  }
+  const size_t nd = (size_t)m * r;
+  if (col_src || col_env) {
+    if (nd > c->desc_cap) {
+      dfree(c->col_src); dfree(c->col_env);
+      c->col_src = c->col_env = nullptr;
+      HIPCK(c, hipMalloc(&c->col_src, sizeof(int) * nd));
+      HIPCK(c, hipMalloc(&c->col_env, sizeof(int) * nd));
+      c->desc_cap = nd;
+    }
+    std::vector<int> src(nd), env(nd);
+    for (size_t i = 0; i < nd; ++i) {
+      src[i] = col_src ? col_src[i] : (int)i;
+      env[i] = col_env ? col_env[i] : -1;
+    }
+    HIPCK(c, hipMemcpyAsync(c->col_src, src.data(), sizeof(int) * nd, hipMemcpyHostToDevice, c->st));
+    HIPCK(c, hipMemcpyAsync(c->col_env, env.data(), sizeof(int) * nd, hipMemcpyHostToDevice, c->st));
+    HIPCK(c, hipStreamSynchronize(c->st));
+    c->have_geno = true;
+    return OAGPU_OK;
+  }
+  // Identity descriptors: kernels read col_src == nullptr as s*r + c.
+  dfree(c->col_src); dfree(c->col_env);
+  c->col_src = c->col_env = nullptr;
+  c->desc_cap = 0;
+  HIPCK(c, hipStreamSynchronize(c->st));
+  c->have_geno = true;
+  return OAGPU_OK;
+}
+
+static int ensure_chunk(oagpu_ctx* c, int mc) {
+  const int r = c->r, l = c->l, p = c->p, G = c->G, Q = c->Q;
+  const size_t arows = (size_t)((mc * r + 127) / 128 * 128);
+  const size_t abytes = arows * c->n_pad * c->op_size;
+  const size_t cbytes = (size_t)mc * r * (c->t + l) * c->c_size;
+  const size_t qbytes = sizeof(double) * (size_t)mc * Q * std::max(G, 1);
+  if (mc > c->mc_cap) {
+    dfree(c->col_mean); dfree(c->Sbr); dfree(c->Str); dfree(c->varx);
+    HIPCK(c, hipMalloc(&c->col_mean, sizeof(double) * (size_t)mc * r));
+    HIPCK(c, hipMalloc(&c->Sbr, sizeof(double) * (size_t)mc * (r * (r + 1) / 2)));
+    HIPCK(c, hipMalloc(&c->Str, sizeof(double) * (size_t)mc * l * r));
+    HIPCK(c, hipMalloc(&c->varx, sizeof(double) * (size_t)mc * p));
+    c->mc_cap = mc;
+  }
+  HIPCK(c, grow(&c->Ahi, &c->Ahi_cap, abytes));
+  if (c->prm.precision == OAGPU_BF16X3) HIPCK(c, grow(&c->Alo, &c->Alo_cap, abytes));
+  HIPCK(c, grow(&c->Cbuf, &c->C_cap, cbytes));
+  HIPCK(c, grow((void**)&c->Ccorr, &c->Ccorr_cap, qbytes));
+  return OAGPU_OK;
+}
+
+extern "C" int oagpu_scan(oagpu_ctx* c, oagpu_dense* dense, oagpu_hit* hits, long long hit_cap,
+                          oagpu_stats* stats) {
+  if (!c) return OAGPU_ERR_ARG;
+  if (!c->have_traits || !c->have_geno) return fail(c, OAGPU_ERR_STATE, "load traits and genotypes before scanning");
+  HIPCK(c, hipSetDevice(c->prm.device));
+  const int r = c->r, p = c->p, t = c->t, m = c->m, l = c->l;
+  // Chunk size: bounded by max_snps and by ~12 GiB of chunk buffers.
+  const size_t per_snp = (size_t)r * c->n_pad * c->op_size * (c->prm.precision == OAGPU_BF16X3 ? 2 : 1) +
+                         (size_t)r * (t + l) * c->c_size + sizeof(double) * (size_t)c->Q * std::max(c->G, 1) +
+                         sizeof(double) * (size_t)(r + r * (r + 1) / 2 + l * r + p);
+  long long mc = c->prm.max_snps > 0 ? c->prm.max_snps : (long long)((12ull << 30) / per_snp);
+  mc = std::max(1ll, std::min(mc, (long long)m));
+  mc = std::min(mc, (long long)(0x7fffffff / std::max(1, r * (t + l))));
+  int rc = ensure_chunk(c, (int)mc);
+  if (rc) return rc;
+
+  // Dense outputs on the device when requested.
+  double *d_beta = nullptr, *d_se = nullptr, *d_t = nullptr, *d_p = nullptr, *d_r2 = nullptr, *d_rss = nullptr,
+         *d_varx = nullptr;
+  uint8_t *d_kept = nullptr, *d_em = nullptr;
+  const size_t npair = (size_t)t * m;
+  const bool want_dense = dense && (dense->beta || dense->se || dense->tstat || dense->pval || dense->kept ||
+                                    dense->r2 || dense->rss || dense->emitted || dense->var_x || dense->n_used);
+  if (want_dense) {
+    HIPCK(c, hipMalloc(&d_beta, sizeof(double) * npair * p));
+    HIPCK(c, hipMalloc(&d_se, sizeof(double) * npair * p));
+    HIPCK(c, hipMalloc(&d_t, sizeof(double) * npair * p));
+    HIPCK(c, hipMalloc(&d_p, sizeof(double) * npair * p));
+    HIPCK(c, hipMalloc(&d_kept, npair * p));
+    HIPCK(c, hipMalloc(&d_r2, sizeof(double) * npair));
+    HIPCK(c, hipMalloc(&d_rss, sizeof(double) * npair));
+    HIPCK(c, hipMalloc(&d_em, npair));
+    HIPCK(c, hipMalloc(&d_varx, sizeof(double) * (size_t)m * p));
+  }
+  for (int i = 0; i < CNT_N; ++i) launch_reset_counter(c->counters, i, c->st);
+
+  const int nchunks = (int)((m + mc - 1) / mc);
+  std::vector<hipEvent_t> ev((size_t)nchunks * 7);
+  for (auto& e : ev) HIPCK(c, hipEventCreate(&e));
+  DevScan d = make_dev(c);
+  d.need_bl = c->prm.disp_cov || want_dense;
+  d.d_beta = d_beta; d.d_se = d_se; d.d_t = d_t; d.d_p = d_p; d.d_kept = d_kept;
+  d.d_r2 = d_r2; d.d_rss = d_rss; d.d_emitted = d_em; d.d_varx = d_varx;
+  for (int ci = 0; ci < nchunks; ++ci) {
+    const int s0 = (int)(ci * mc);
+    const int mcc = (int)std::min<long long>(mc, m - s0);
+    d.s0 = s0;
+    d.mc = mcc;
+    hipEvent_t* E = &ev[(size_t)ci * 7];
+    HIPCK(c, hipEventRecord(E[0], c->st));
+    launch_prep(d, c->prm.precision, c->Ahi, c->Alo, c->st);
+    HIPCK(c, hipEventRecord(E[1], c->st));
+    launch_gemm(c->prm.precision, c->Ahi, c->Alo, c->Bhi, c->Blo, c->Cbuf, mcc * r, t + l, c->n_pad, t + l,
+                c->st);
+    HIPCK(c, hipEventRecord(E[2], c->st));
+    launch_snp_gram(d, c->prm.precision, c->lb, c->st);
+    HIPCK(c, hipEventRecord(E[3], c->st));
+    if (c->max_pat_cnt > 0) launch_mask(d, c->lb, c->st);
+    HIPCK(c, hipEventRecord(E[4], c->st));
+    launch_reset_counter(c->counters, CNT_FB_CHUNK, c->st);
+    launch_epilogue(d, c->prm.precision, c->lb, c->st);
+    HIPCK(c, hipEventRecord(E[5], c->st));
+    launch_fallback(d, c->lb, c->st);
+    HIPCK(c, hipEventRecord(E[6], c->st));
+    HIPCK(c, hipGetLastError());
+  }
+  unsigned long long cnt[CNT_N];
+  HIPCK(c, hipMemcpyAsync(cnt, c->counters, sizeof(cnt), hipMemcpyDeviceToHost, c->st));
+  HIPCK(c, hipStreamSynchronize(c->st));
+  for (int i = 0; i < 8; ++i) c->timings[i] = 0.0;
+  for (int ci = 0; ci < nchunks; ++ci) {
+    hipEvent_t* E = &ev[(size_t)ci * 7];
+    for (int k = 0; k < 6; ++k) {
+      float ms = 0.f;
+      HIPCK(c, hipEventElapsedTime(&ms, E[k], E[k + 1]));
+      c->timings[k] += ms;
+    }
+  }
+  {
+    float ms = 0.f;
+    HIPCK(c, hipEventElapsedTime(&ms, ev[0], ev[(size_t)(nchunks - 1) * 7 + 6]));
+    c->timings[6] = ms;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+
+  const long long nh = (long long)cnt[CNT_HITS];
+  if (stats) {
+    stats->n_hits = nh;
+    stats->n_emitted = (long long)cnt[CNT_EMITTED];
+    stats->n_sig = (long long)cnt[CNT_SIG];
+    stats->n_fallback = (long long)cnt[CNT_FALLBACK];
+    stats->n_patterns = c->G;
+    stats->n_chunks = nchunks;
+    stats->t_star = oagpu_tvalue(c->prm.min_p_store);
+  }
+  int status = OAGPU_OK;
+  if (cnt[CNT_ERR] & 1ull)
+    status = fail(c, OAGPU_ERR_NUMERIC, "Error with Syy < 0 or var_x < 0 (the reference exits here)");
+  if (want_dense) {
+    auto cp = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
+      if (!dst) return hipSuccess;
+      return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+    };
+    HIPCK(c, cp(dense->beta, d_beta, sizeof(double) * npair * p));
+    HIPCK(c, cp(dense->se, d_se, sizeof(double) * npair * p));
+    HIPCK(c, cp(dense->tstat, d_t, sizeof(double) * npair * p));
+    HIPCK(c, cp(dense->pval, d_p, sizeof(double) * npair * p));
+    HIPCK(c, cp(dense->kept, d_kept, npair * p));
+    HIPCK(c, cp(dense->r2, d_r2, sizeof(double) * npair));
+    HIPCK(c, cp(dense->rss, d_rss, sizeof(double) * npair));
+    HIPCK(c, cp(dense->emitted, d_em, npair));
+    HIPCK(c, cp(dense->var_x, d_varx, sizeof(double) * (size_t)m * p));
+    HIPCK(c, cp(dense->n_used, c->ymiss, sizeof(int) * t));
+    if (dense->n_used)
+      for (int y = 0; y < t; ++y) dense->n_used[y] = c->n - dense->n_used[y];
+    void* bufs[] = {d_beta, d_se, d_t, d_p, d_kept, d_r2, d_rss, d_em, d_varx};
+    for (void* b : bufs) dfree(b);
+  }
+  if (nh > c->hit_cap || (hits && nh > hit_cap)) {
+    return fail(c, OAGPU_ERR_HIT_OVERFLOW,
+                "hit arena too small: " + std::to_string(nh) + " hits (capacity " +
+                    std::to_string(std::min(c->hit_cap, hits ? hit_cap : c->hit_cap)) + ")");
+  }
+  if (hits && nh > 0) {
+    HIPCK(c, hipMemcpy(hits, c->hits, sizeof(oagpu_hit) * (size_t)nh, hipMemcpyDeviceToHost));
+    std::sort(hits, hits + nh, [](const oagpu_hit& a, const oagpu_hit& b) {
+      if (a.trait != b.trait) return a.trait < b.trait;
+      if (a.snp != b.snp) return a.snp < b.snp;
+      return a.h > b.h;
+    });
+  }
+  return status;
+}
+
+extern "C" int oagpu_last_timings(const oagpu_ctx* c, double* out, int n) {
+  if (!c || !out) return 0;
+  const int k = std::min(n, 7);
+  for (int i = 0; i < k; ++i) out[i] = c->timings[i];
+  return k;
+}

@@ -1,0 +1,89 @@
+// Shared device helpers for the gfx950 scan kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace oa {
+
+enum { PREC_FP64 = 0, PREC_FP32 = 1, PREC_BF16X3 = 2 };
+enum { DT_F64 = 0, DT_F32 = 1, DT_F16 = 2 };
+
+// Largest coefficient count a pair can have (l + r), and the stride of a
+// coefficient vector in the fallback list.
+constexpr int PMAX = 40;
+
+// Relative size below which the closed-form RSS is replaced by an explicit
+// residual pass: RSS < RSS_GUARD * y'y.
+constexpr double RSS_GUARD = 1e-4;
+
+typedef __attribute__((ext_vector_type(4))) double d4;
+typedef __attribute__((ext_vector_type(4))) float f4;
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+
+__device__ __forceinline__ double load_elem(const void* base, int dt, size_t idx) {
+  if (dt == DT_F64) return reinterpret_cast<const double*>(base)[idx];
+  if (dt == DT_F32) return (double)reinterpret_cast<const float*>(base)[idx];
+  return (double)(float)reinterpret_cast<const _Float16*>(base)[idx];
+}
+
+// float -> bf16 bits, round to nearest even (finite inputs).
+__device__ __forceinline__ uint16_t bf16_bits_rn(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_bits_to_float(uint16_t b) {
+  return __uint_as_float(((uint32_t)b) << 16);
+}
+// Two-term split x ~= hi + lo with hi = bf16(x), lo = bf16(x - hi).
+__device__ __forceinline__ void split_bf16(float x, uint16_t& hi, uint16_t& lo) {
+  hi = bf16_bits_rn(x);
+  lo = bf16_bits_rn(x - bf16_bits_to_float(hi));
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ long long wave_sum_ll(long long v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Block-wide sum for blockDim.x == 256 (4 waves).  All threads get the sum.
+__device__ __forceinline__ double block_sum256(double v, double* sh /*[4]*/) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double s = sh[0] + sh[1] + sh[2] + sh[3];
+  return s;
+}
+__device__ __forceinline__ long long block_sum256_ll(long long v, long long* sh) {
+  v = wave_sum_ll(v);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  return sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// Packed upper-triangular index of (row i, col j), i <= j.
+__host__ __device__ __forceinline__ int upk(int i, int j) { return j * (j + 1) / 2 + i; }
+
+__device__ __forceinline__ uint64_t mix64a(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+__device__ __forceinline__ uint64_t mix64b(uint64_t x) {
+  x = (x + 0x632BE59BD9B4E019ull) * 0xD6E8FEB86659FD93ull;
+  x = (x ^ (x >> 32)) * 0xD6E8FEB86659FD93ull;
+  return x ^ (x >> 32);
+}
+
+}  // namespace oa

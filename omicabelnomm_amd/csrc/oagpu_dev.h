@@ -1,0 +1,91 @@
+// Device-side view of one scan: every pointer a kernel reads or writes.
+// Passed to kernels by value.
+#pragma once
+#include "oagpu_common.h"
+#include "../../include/oagpu.h"
+
+namespace oa {
+
+// One pair whose RSS needs the explicit residual pass.
+struct FallbackEntry {
+  int y;        // local trait
+  int s;        // SNP within the chunk
+  int flags;    // bit0: b is the untouched right-hand side (failed Cholesky)
+  int pad;
+  double b[PMAX];
+};
+
+enum {
+  CNT_HITS = 0,      // coefficients reserved in the hit arena
+  CNT_EMITTED = 1,   // emitted (trait, SNP) records
+  CNT_SIG = 2,       // kept coefficients with P < minPdisp
+  CNT_FALLBACK = 3,  // pairs sent to the explicit residual pass (all chunks)
+  CNT_ERR = 4,       // bit0: RSS<0 / var_x<0 (reference exit); bit1: fallback overflow
+  CNT_FB_CHUNK = 5,  // fallback entries of the current chunk
+  CNT_N = 8
+};
+
+struct DevScan {
+  // shapes
+  int n, n_pad, l, r, p, Q;   // Q = l*r + r*(r+1)/2 correction terms
+  int t, ldt;                 // traits, stride of per-trait arrays
+  int ldc;                    // row stride of the contraction output (>= t + l)
+  int G;                      // missing-value patterns
+  int start_h;
+  int disp_cov, store_all, need_bl;
+  double min_r2_store, t_star, min_p_disp;
+
+  // covariates
+  const double* XL_cm;   // [l][n_pad] column-major
+  const double* XL_rm;   // [n][l] row-major
+  const double* U0;      // packed chol(Stl) (l(l+1)/2)
+  const double* Stl_inv_diag;  // diag(Stl^-1) (l)
+
+  // interactions
+  const double* E;       // [n_env][lde]
+  long long lde;
+
+  // traits
+  const void* Y; int ydt; long long ldy;
+  const int* ymiss;      // [t]
+  const int* yzero;      // [t] observed values that are exactly 0
+  const double* yy;      // [t] sum y0^2
+  const double* sst;     // [t]
+  const double* rss0;    // [t] y0'y0 - |zt|^2
+  const int* ypat;       // [t] pattern id
+  const double* rt;      // [l][ldt] XL'y0
+  const double* zt;      // [l][ldt] U_A^-T XL'y0
+
+  // patterns
+  const int* pat_cnt;    // [G] missing rows
+  const int* pat_rep;    // [G] representative trait; its rows are at rows + rep*n
+  const int* rows;       // sorted missing rows of trait y at rows + y*n
+  const double* UA;      // [l(l+1)/2][G] packed chol(Stl - C_ll)
+  const double* UA_invd; // [l][G] 1/diag
+  const int* pat_pd;     // [G] 1 if Stl - C_ll is positive definite
+
+  // genotypes
+  const void* Gm; int gdt; long long ldg;
+  const int* col_src; const int* col_env;   // [m*r] for the whole loaded block
+
+  // chunk
+  int s0, mc;            // first SNP of the chunk, SNPs in the chunk
+  double* col_mean;      // [mc*r]
+  double* Sbr;           // [mc][r(r+1)/2] packed upper
+  double* Str;           // [mc][l][r]
+  double* varx;          // [mc][p]
+  double* Ccorr;         // [mc][Q][G]
+  const void* Cbuf;      // [mc*r][ldc]
+
+  // outputs
+  oagpu_hit* hits; long long hit_cap;
+  unsigned long long* counters;
+  int trait_offset, snp_offset;
+  int m_total;           // SNPs in the loaded block (dense stride)
+  double *d_beta, *d_se, *d_t, *d_p; uint8_t* d_kept;
+  double *d_r2, *d_rss; uint8_t* d_emitted; double* d_varx;
+
+  FallbackEntry* fb; int fb_cap;
+};
+
+}  // namespace oa

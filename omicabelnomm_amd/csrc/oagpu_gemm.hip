@@ -1,0 +1,214 @@
+// The bulk contraction of the scan, C = A' [Y0 | XL], on gfx950 MFMA.
+// It replaces the reference's BLAS calls
+//   Ay_bot = XR'Y  (cblas_sgemm, src/Algorithm.cpp:705-707)
+//   Str    = XL'XR (cblas_sgemm, src/Algorithm.cpp:633-635)
+// with one pass over the genotype operand: the covariate columns ride along
+// as extra columns of B.
+//
+// Layout: A has M rows (genotype columns) of K contiguous values, B has N
+// rows (traits, then covariates) of K contiguous values; both are padded to
+// whole 128-row tiles and K is a multiple of 64 (zero rows).  C is row-major
+// M x ldc.  One workgroup = 4 waves = a 128 x 128 tile, each wave 64 x 64
+// (4 x 4 MFMA blocks of 16 x 16).  Each lane feeds the MFMAs straight from
+// global memory: lane (i, g) = (lane & 15, lane >> 4) reads a contiguous
+// run of K values of row i, so the 4 lane groups of a row read one full
+// cache line.  The k-values a lane group supplies are chosen identically
+// for A and B, so the contraction is exact whatever the hardware's internal
+// k order.
+//
+// Workgroups are mapped to tiles XCD-major: the hardware hands consecutive
+// workgroup ids to the 8 XCDs round-robin, and each XCD receives a
+// contiguous range of tiles (row-tile major), so the tiles that share an A
+// row block run on the same XCD and reuse it from that XCD's L2.
+#include "oagpu_kernels.h"
+
+namespace oa {
+
+__device__ __forceinline__ int xcd_tile(int id, int total) {
+  const int q = total >> 3, rem = total & 7;
+  const int xcd = id & 7, loc = id >> 3;
+  return xcd * q + (xcd < rem ? xcd : rem) + loc;
+}
+
+// FP64: v_mfma_f64_16x16x4f64.  K step 16 = 4 MFMA steps; lane group g
+// supplies k = 4g + step.
+__global__ __launch_bounds__(256, 2) void k_gemm_f64(const double* __restrict__ A,
+                                                    const double* __restrict__ B,
+                                                    double* __restrict__ C, int M, int N, int K,
+                                                    int ldc, int tiles_n, int total) {
+  const int tile = xcd_tile(blockIdx.x, total);
+  const int tm = tile / tiles_n, tn = tile - tm * tiles_n;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = lane & 15, g = lane >> 4;
+  const int row0 = tm * 128 + (w & 1) * 64;
+  const int col0 = tn * 128 + (w >> 1) * 64;
+  d4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+  const double* Ap[4];
+  const double* Bp[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    Ap[a] = A + (size_t)(row0 + 16 * a + i) * K + 4 * g;
+    Bp[a] = B + (size_t)(col0 + 16 * a + i) * K + 4 * g;
+  }
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    d4 av[4], bv[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      av[a] = *reinterpret_cast<const d4*>(Ap[a] + k0);
+      bv[a] = *reinterpret_cast<const d4*>(Bp[a] + k0);
+    }
+#pragma unroll
+    for (int st = 0; st < 4; ++st)
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a][st], bv[b][st], acc[a][b], 0, 0, 0);
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int col = col0 + 16 * b + i;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        // f64 16x16x4 output layout: lane (i, g), value v -> row 4v + g
+        const int row = row0 + 16 * a + 4 * v + g;
+        if (row < M && col < N) C[(size_t)row * ldc + col] = acc[a][b][v];
+      }
+    }
+}
+
+// FP32: v_mfma_f32_16x16x4f32, same structure with 4-float runs.
+__global__ __launch_bounds__(256, 2) void k_gemm_f32(const float* __restrict__ A,
+                                                    const float* __restrict__ B,
+                                                    float* __restrict__ C, int M, int N, int K,
+                                                    int ldc, int tiles_n, int total) {
+  const int tile = xcd_tile(blockIdx.x, total);
+  const int tm = tile / tiles_n, tn = tile - tm * tiles_n;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = lane & 15, g = lane >> 4;
+  const int row0 = tm * 128 + (w & 1) * 64;
+  const int col0 = tn * 128 + (w >> 1) * 64;
+  f4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+  const float* Ap[4];
+  const float* Bp[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    Ap[a] = A + (size_t)(row0 + 16 * a + i) * K + 4 * g;
+    Bp[a] = B + (size_t)(col0 + 16 * a + i) * K + 4 * g;
+  }
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    f4 av[4], bv[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      av[a] = *reinterpret_cast<const f4*>(Ap[a] + k0);
+      bv[a] = *reinterpret_cast<const f4*>(Bp[a] + k0);
+    }
+#pragma unroll
+    for (int st = 0; st < 4; ++st)
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[a][st], bv[b][st], acc[a][b], 0, 0, 0);
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int col = col0 + 16 * b + i;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = row0 + 16 * a + 4 * g + v;
+        if (row < M && col < N) C[(size_t)row * ldc + col] = acc[a][b][v];
+      }
+    }
+}
+
+// Split BF16, 3 terms: x*y ~= xh*yh + xh*yl + xl*yh with FP32 accumulation,
+// v_mfma_f32_16x16x32_bf16.  K step 32; lane group g supplies k = 8g..8g+7.
+typedef __attribute__((ext_vector_type(4))) unsigned int u4;
+
+__device__ __forceinline__ bf16x8 as_bf16x8(u4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+__global__ __launch_bounds__(256, 2) void k_gemm_bf16x3(const uint16_t* __restrict__ Ah,
+                                                       const uint16_t* __restrict__ Al,
+                                                       const uint16_t* __restrict__ Bh,
+                                                       const uint16_t* __restrict__ Bl,
+                                                       float* __restrict__ C, int M, int N, int K,
+                                                       int ldc, int tiles_n, int total) {
+  const int tile = xcd_tile(blockIdx.x, total);
+  const int tm = tile / tiles_n, tn = tile - tm * tiles_n;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = lane & 15, g = lane >> 4;
+  const int row0 = tm * 128 + (w & 1) * 64;
+  const int col0 = tn * 128 + (w >> 1) * 64;
+  f4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+  size_t ao[4], bo[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    ao[a] = (size_t)(row0 + 16 * a + i) * K + 8 * g;
+    bo[a] = (size_t)(col0 + 16 * a + i) * K + 8 * g;
+  }
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    u4 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      ah[a] = *reinterpret_cast<const u4*>(Ah + ao[a] + k0);
+      al[a] = *reinterpret_cast<const u4*>(Al + ao[a] + k0);
+      bh[a] = *reinterpret_cast<const u4*>(Bh + bo[a] + k0);
+      bl[a] = *reinterpret_cast<const u4*>(Bl + bo[a] + k0);
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(al[a]), as_bf16x8(bh[b]), acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(ah[a]), as_bf16x8(bl[b]), acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(ah[a]), as_bf16x8(bh[b]), acc[a][b], 0, 0, 0);
+      }
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int col = col0 + 16 * b + i;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = row0 + 16 * a + 4 * g + v;
+        if (row < M && col < N) C[(size_t)row * ldc + col] = acc[a][b][v];
+      }
+    }
+}
+
+void launch_gemm(int prec, const void* Ahi, const void* Alo, const void* Bhi, const void* Blo,
+                 void* C, int M, int N, int K, int ldc, hipStream_t st) {
+  const int tiles_m = (M + 127) / 128, tiles_n = (N + 127) / 128;
+  const int total = tiles_m * tiles_n;
+  dim3 grid(total), block(256);
+  if (prec == PREC_FP64)
+    hipLaunchKernelGGL(k_gemm_f64, grid, block, 0, st, (const double*)Ahi, (const double*)Bhi,
+                       (double*)C, M, N, K, ldc, tiles_n, total);
+  else if (prec == PREC_FP32)
+    hipLaunchKernelGGL(k_gemm_f32, grid, block, 0, st, (const float*)Ahi, (const float*)Bhi,
+                       (float*)C, M, N, K, ldc, tiles_n, total);
+  else
+    hipLaunchKernelGGL(k_gemm_bf16x3, grid, block, 0, st, (const uint16_t*)Ahi, (const uint16_t*)Alo,
+                       (const uint16_t*)Bhi, (const uint16_t*)Blo, (float*)C, M, N, K, ldc, tiles_n,
+                       total);
+}
+
+}  // namespace oa

@@ -1,0 +1,756 @@
+// SNP-side kernels of one chunk.  They replace, per SNP block:
+//   replace_nans + replace_nans_avgs on XR .. src/Algorithm.cpp:618-622, src/Utility.cpp:103-127
+//   interaction synthesis .................... src/AIOwrapper.cpp:767-840 (fused into the loads)
+//   Sbr = XR_s'XR_s, build_S, potrf/potri .... src/Algorithm.cpp:646-671
+//   generate_correction_missings_Str ......... src/Algorithm.cpp:1260-1315
+//   per-pair build_S/correct_missings_S/sposv  src/Algorithm.cpp:714-792, 1370-1399
+//   hpc_SSY (RSS) ............................ src/Algorithm.cpp:896-1024
+//   hpc_statistics ........................... src/Algorithm.cpp:1034-1214
+// Everything here is FP64.  The pair solve uses the block structure of
+// S_pair = [[A_g, B],[B', D]]: A_g (covariates, per missing pattern) is
+// factored once per pattern, each pair factors only its r x r Schur
+// complement, and RSS is the downdate RSS0(trait) - |z_r|^2 of the
+// covariate-only residual, plus the terms the reference's misplaced
+// corrections add.  In exact arithmetic this is the reference's
+// full-matrix Cholesky solve (same pivots, same failure rule).
+#include "oagpu_kernels.h"
+
+namespace oa {
+
+template <int R>
+__device__ __forceinline__ void col_desc(const DevScan& d, int gs, int* src, int* env) {
+#pragma unroll
+  for (int c = 0; c < R; ++c) {
+    const int idx = gs * R + c;
+    src[c] = d.col_src ? d.col_src[idx] : idx;
+    env[c] = d.col_env ? d.col_env[idx] : -1;
+  }
+}
+
+// Genotype value of output column (src, env) at row k, NaN if missing.
+__device__ __forceinline__ double geno_value(const DevScan& d, int src, int env, int k) {
+  double v = load_elem(d.Gm, d.gdt, (size_t)src * d.ldg + k);
+  if (env >= 0) v *= d.E[(size_t)env * d.lde + k];
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// One block per SNP: mean imputation, interaction columns, the contraction
+// operand A (column s*R+c, rows n..n_pad-1 zero) and Sbr = XR_s'XR_s.
+template <int R, int OP>
+__global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo) {
+  const int s = blockIdx.x;
+  const int gs = d.s0 + s;
+  const int tid = threadIdx.x;
+  constexpr int RR = R * (R + 1) / 2;
+  __shared__ double shd[4];
+  int src[R], env[R];
+  col_desc<R>(d, gs, src, env);
+
+  double sum[R], cnt[R];
+#pragma unroll
+  for (int c = 0; c < R; ++c) { sum[c] = 0.0; cnt[c] = 0.0; }
+  for (int k = tid; k < d.n; k += 256) {
+#pragma unroll
+    for (int c = 0; c < R; ++c) {
+      const double v = geno_value(d, src[c], env[c], k);
+      if (v == v) sum[c] += v;
+      else cnt[c] += 1.0;
+    }
+  }
+  double mean[R];
+#pragma unroll
+  for (int c = 0; c < R; ++c) {
+    const double sc = block_sum256(sum[c], shd);
+    const double nc = block_sum256(cnt[c], shd);
+    mean[c] = sc / ((double)d.n - nc);
+  }
+
+  double acc[RR];
+#pragma unroll
+  for (int e = 0; e < RR; ++e) acc[e] = 0.0;
+  for (int k = tid; k < d.n_pad; k += 256) {
+    double x[R];
+#pragma unroll
+    for (int c = 0; c < R; ++c) {
+      double v = 0.0;
+      if (k < d.n) {
+        v = geno_value(d, src[c], env[c], k);
+        if (v != v) v = mean[c];
+      }
+      x[c] = v;
+      const size_t o = (size_t)(s * R + c) * d.n_pad + k;
+      if (OP == PREC_FP64) {
+        reinterpret_cast<double*>(Ahi)[o] = v;
+      } else if (OP == PREC_FP32) {
+        reinterpret_cast<float*>(Ahi)[o] = (float)v;
+      } else {
+        uint16_t hi, lo;
+        split_bf16((float)v, hi, lo);
+        reinterpret_cast<uint16_t*>(Ahi)[o] = hi;
+        reinterpret_cast<uint16_t*>(Alo)[o] = lo;
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < R; ++b)
+#pragma unroll
+      for (int a = 0; a <= b; ++a) acc[upk(a, b)] += x[a] * x[b];
+  }
+#pragma unroll
+  for (int e = 0; e < RR; ++e) {
+    const double v = block_sum256(acc[e], shd);
+    if (tid == 0) d.Sbr[(size_t)s * RR + e] = v;
+  }
+  if (tid == 0) {
+#pragma unroll
+    for (int c = 0; c < R; ++c) d.col_mean[(size_t)s * R + c] = mean[c];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// One thread per SNP: Str from the contraction output, and the unmasked
+// var_x = diag(S_full^-1) (src/Algorithm.cpp:659-671) through the block
+// inverse with U0 = chol(Stl).
+template <int R, int LB, typename CT>
+__global__ void k_snp_gram(DevScan d) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= d.mc) return;
+  const int l = d.l, p = d.p;
+  constexpr int RR = R * (R + 1) / 2;
+  const CT* C = reinterpret_cast<const CT*>(d.Cbuf);
+  double X[LB][R];
+#pragma unroll
+  for (int j = 0; j < LB; ++j)
+#pragma unroll
+    for (int c = 0; c < R; ++c) {
+      X[j][c] = 0.0;
+      if (j < l) {
+        const double v = (double)C[(size_t)(s * R + c) * d.ldc + d.t + j];
+        d.Str[((size_t)s * l + j) * R + c] = v;
+        X[j][c] = v;
+      }
+    }
+  // X = U0^-T Str
+#pragma unroll
+  for (int j = 0; j < LB; ++j) {
+    if (j < l) {
+      const double inv = 1.0 / d.U0[upk(j, j)];
+#pragma unroll
+      for (int c = 0; c < R; ++c) {
+        double v = X[j][c];
+#pragma unroll
+        for (int i = 0; i < j; ++i) v -= d.U0[upk(i, j)] * X[i][c];
+        X[j][c] = v * inv;
+      }
+    }
+  }
+  // Schur0 = Sbr - X'X, then its Cholesky factor.
+  double S[RR];
+#pragma unroll
+  for (int b = 0; b < R; ++b)
+#pragma unroll
+    for (int a = 0; a <= b; ++a) {
+      double v = d.Sbr[(size_t)s * RR + upk(a, b)];
+#pragma unroll
+      for (int j = 0; j < LB; ++j)
+        if (j < l) v -= X[j][a] * X[j][b];
+      S[upk(a, b)] = v;
+    }
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    double ajj = S[upk(j, j)];
+#pragma unroll
+    for (int k = 0; k < j; ++k) ajj -= S[upk(k, j)] * S[upk(k, j)];
+    if (!(ajj > 0.0)) ok = false;
+    ajj = sqrt(ajj);
+    S[upk(j, j)] = ajj;
+#pragma unroll
+    for (int i = j + 1; i < R; ++i) {
+      double aji = S[upk(j, i)];
+#pragma unroll
+      for (int k = 0; k < j; ++k) aji -= S[upk(k, j)] * S[upk(k, i)];
+      S[upk(j, i)] = aji / ajj;
+    }
+  }
+  double* vx = d.varx + (size_t)s * p;
+  if (!ok) {
+    for (int h = 0; h < p; ++h) vx[h] = __longlong_as_double(0x7ff8000000000000ll);
+  } else {
+    // Ui = U_S^-1 (upper), Sinv = Ui Ui'
+    double Ui[R][R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+#pragma unroll
+      for (int i = 0; i < R; ++i) Ui[i][j] = 0.0;
+      Ui[j][j] = 1.0 / S[upk(j, j)];
+#pragma unroll
+      for (int i = j - 1; i >= 0; --i) {
+        double v = 0.0;
+#pragma unroll
+        for (int k = i + 1; k <= j; ++k) v += S[upk(i, k)] * Ui[k][j];
+        Ui[i][j] = -v / S[upk(i, i)];
+      }
+    }
+    double Sinv[R][R];
+#pragma unroll
+    for (int a = 0; a < R; ++a)
+#pragma unroll
+      for (int b = 0; b < R; ++b) {
+        double v = 0.0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) v += Ui[a][k] * Ui[b][k];
+        Sinv[a][b] = v;
+      }
+#pragma unroll
+    for (int a = 0; a < R; ++a) vx[l + a] = Sinv[a][a];
+    // W = U0^-1 X (in place, back substitution)
+#pragma unroll
+    for (int j = LB - 1; j >= 0; --j) {
+      if (j < l) {
+        const double inv = 1.0 / d.U0[upk(j, j)];
+#pragma unroll
+        for (int c = 0; c < R; ++c) {
+          double v = X[j][c];
+#pragma unroll
+          for (int i = j + 1; i < LB; ++i)
+            if (i < l) v -= d.U0[upk(j, i)] * X[i][c];
+          X[j][c] = v * inv;
+        }
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < LB; ++h) {
+      if (h < l) {
+        double v = d.Stl_inv_diag[h];
+#pragma unroll
+        for (int a = 0; a < R; ++a)
+#pragma unroll
+          for (int b = 0; b < R; ++b) v += X[h][a] * Sinv[a][b] * X[h][b];
+        vx[h] = v;
+      }
+    }
+  }
+  if (d.d_varx)
+    for (int h = 0; h < p; ++h) d.d_varx[(size_t)(d.s0 + s) * p + h] = vx[h];
+}
+
+// ---------------------------------------------------------------------------
+// Missing-row corrections per (pattern, SNP): C_lr = sum_M xl x_s' (l x r)
+// and C_rr = sum_M x_s x_s' (upper), over the pattern's missing rows.  One
+// wave per pattern, one lane per SNP; the row index and covariate row are
+// wave-uniform.
+template <int R, int LB>
+__global__ __launch_bounds__(256) void k_mask(DevScan d) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int g = blockIdx.y * 4 + w;
+  if (g >= d.G) return;
+  const int s = blockIdx.x * 64 + lane;
+  const int se = s < d.mc ? s : d.mc - 1;
+  const int l = d.l;
+  constexpr int RR = R * (R + 1) / 2;
+  int src[R], env[R];
+  col_desc<R>(d, d.s0 + se, src, env);
+  double mean[R];
+#pragma unroll
+  for (int c = 0; c < R; ++c) mean[c] = d.col_mean[(size_t)se * R + c];
+  double lr[LB][R], rr[RR];
+#pragma unroll
+  for (int j = 0; j < LB; ++j)
+#pragma unroll
+    for (int c = 0; c < R; ++c) lr[j][c] = 0.0;
+#pragma unroll
+  for (int e = 0; e < RR; ++e) rr[e] = 0.0;
+  const int cnt = d.pat_cnt[g];
+  const int* rows = d.rows + (size_t)d.pat_rep[g] * d.n;
+  for (int q = 0; q < cnt; ++q) {
+    const int k = rows[q];
+    double x[R];
+#pragma unroll
+    for (int c = 0; c < R; ++c) {
+      double v = geno_value(d, src[c], env[c], k);
+      x[c] = (v == v) ? v : mean[c];
+    }
+    const double* xl = d.XL_rm + (size_t)k * l;
+#pragma unroll
+    for (int j = 0; j < LB; ++j)
+      if (j < l) {
+        const double a = xl[j];
+#pragma unroll
+        for (int c = 0; c < R; ++c) lr[j][c] += a * x[c];
+      }
+#pragma unroll
+    for (int b = 0; b < R; ++b)
+#pragma unroll
+      for (int a = 0; a <= b; ++a) rr[upk(a, b)] += x[a] * x[b];
+  }
+  if (s < d.mc) {
+    const size_t base = (size_t)s * d.Q;
+#pragma unroll
+    for (int j = 0; j < LB; ++j)
+      if (j < l)
+#pragma unroll
+        for (int c = 0; c < R; ++c) d.Ccorr[(base + j * R + c) * d.G + g] = lr[j][c];
+#pragma unroll
+    for (int e = 0; e < RR; ++e) d.Ccorr[(base + l * R + e) * d.G + g] = rr[e];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// hpc_statistics for one (trait, SNP) pair (src/Algorithm.cpp:1034-1214):
+// R2, SE from the unmasked var_x scaled by (n - Ymiss)/n, t, the
+// normal-approximation P, thresholds, counts, and the hit records.
+__device__ __forceinline__ double pvalue_ref(double t) {
+  if (t < 1.6) return 0.5 - 0.1 * (4.4 - t) * t;
+  return erfc(t * 0.70710678118654752440084436210484903928);
+}
+
+template <int P>
+__device__ void stats_emit(const DevScan& d, int y, int s, const double* b, double rss) {
+  const int l = d.l, p = d.p;
+  const int ymiss = d.ymiss[y];
+  const int n_c = d.n - ymiss - p;
+  const double vf = (double)(n_c + p) / (double)d.n;
+  const double R2 = 1.0 - ((rss / (double)(n_c - 1)) / (d.sst[y] / (double)(d.n - 1)));
+  const double* vx = d.varx + (size_t)s * p;
+  const size_t pair = (size_t)y * d.m_total + (size_t)(d.s0 + s);
+  const bool dense = d.d_beta != nullptr;
+  const double qnan = __longlong_as_double(0x7ff8000000000000ll);
+  if (dense) {
+    d.d_r2[pair] = R2;
+    d.d_rss[pair] = rss;
+#pragma unroll
+    for (int h = 0; h < P; ++h)
+      if (h < p) {
+        d.d_beta[pair * p + h] = b[h];
+        d.d_se[pair * p + h] = qnan;
+        d.d_t[pair * p + h] = qnan;
+        d.d_p[pair * p + h] = qnan;
+        d.d_kept[pair * p + h] = 0;
+      }
+  }
+  unsigned long long mask = 0;
+  int nk = 0, nsig = 0;
+  bool has_snp = false;
+  double sig = 0.0;
+  if (rss > 0.0 && (R2 > d.min_r2_store || d.store_all)) {
+    sig = sqrt(rss / (double)n_c);
+#pragma unroll
+    for (int h = P - 1; h >= 0; --h) {
+      if (h < p && h >= d.start_h) {
+        const double vxh = vx[h];
+        if (vxh < 0.0) atomicOr(&d.counters[CNT_ERR], 1ull);
+        const double se = sig * sqrt(vxh * vf);
+        const double tv = fabs(b[h] / se);
+        if (dense) {
+          d.d_se[pair * p + h] = se;
+          d.d_t[pair * p + h] = tv;
+        }
+        if (tv > d.t_star || d.store_all) {
+          mask |= 1ull << h;
+          ++nk;
+          if (h >= l) has_snp = true;
+          const double pv = pvalue_ref(tv);
+          if (d.min_p_disp > pv) ++nsig;
+          if (dense) {
+            d.d_p[pair * p + h] = pv;
+            d.d_kept[pair * p + h] = 1;
+          }
+        }
+      }
+    }
+  }
+  if (nsig) atomicAdd(&d.counters[CNT_SIG], (unsigned long long)nsig);
+  if (dense) d.d_emitted[pair] = has_snp ? 1 : 0;
+  if (!has_snp) return;
+  atomicAdd(&d.counters[CNT_EMITTED], 1ull);
+  const unsigned long long base = atomicAdd(&d.counters[CNT_HITS], (unsigned long long)nk);
+  if (base + nk > (unsigned long long)d.hit_cap) return;
+  unsigned long long o = base;
+#pragma unroll
+  for (int h = P - 1; h >= 0; --h) {
+    if (h < p && ((mask >> h) & 1ull)) {
+      const double se = sig * sqrt(vx[h] * vf);
+      const double tv = fabs(b[h] / se);
+      oagpu_hit hr;
+      hr.trait = d.trait_offset + y;
+      hr.snp = d.snp_offset + d.s0 + s;
+      hr.h = h;
+      hr.n_used = d.n - ymiss;
+      hr.b = b[h];
+      hr.se = se;
+      hr.t = tv;
+      hr.p = pvalue_ref(tv);
+      hr.r2 = R2;
+      d.hits[o++] = hr;
+    }
+  }
+}
+
+// RSS over rows with y0 != 0 (src/Algorithm.cpp:976-1024), one thread.
+template <int R, int LB>
+__device__ double explicit_rss_thread(const DevScan& d, int y, int s, const double* b) {
+  int src[R], env[R];
+  col_desc<R>(d, d.s0 + s, src, env);
+  double mean[R];
+#pragma unroll
+  for (int c = 0; c < R; ++c) mean[c] = d.col_mean[(size_t)s * R + c];
+  double acc = 0.0;
+  for (int k = 0; k < d.n; ++k) {
+    double yv = load_elem(d.Y, d.ydt, (size_t)y * d.ldy + k);
+    if (yv != yv) yv = 0.0;
+    if (yv == 0.0) continue;
+    double sy = 0.0;
+#pragma unroll
+    for (int h = 0; h < LB; ++h)
+      if (h < d.l) sy += d.XL_cm[(size_t)h * d.n_pad + k] * b[h];
+#pragma unroll
+    for (int c = 0; c < R; ++c) {
+      double v = geno_value(d, src[c], env[c], k);
+      if (v != v) v = mean[c];
+      sy += v * b[d.l + c];
+    }
+    sy -= yv;
+    acc += sy * sy;
+  }
+  return acc;
+}
+
+// ---------------------------------------------------------------------------
+// One thread per (trait, SNP) pair: a wave covers 64 traits of one SNP, so
+// the SNP's Gram pieces are wave-uniform.
+template <int R, int LB, typename CT>
+__global__ __launch_bounds__(256) void k_epilogue(DevScan d) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int s = blockIdx.y * 4 + w;
+  if (s >= d.mc) return;
+  const int y = blockIdx.x * 64 + lane;
+  if (y >= d.t) return;
+  constexpr int RR = R * (R + 1) / 2;
+  constexpr int P = LB + R;
+  const int l = d.l, G = d.G;
+  const int g = d.ypat[y];
+  const int cnt = d.pat_cnt[g];
+  const CT* C = reinterpret_cast<const CT*>(d.Cbuf);
+
+  double rb[R];
+#pragma unroll
+  for (int c = 0; c < R; ++c) rb[c] = (double)C[(size_t)(s * R + c) * d.ldc + y];
+
+  // Off-diagonal block B_q as the reference assembles it: every column's
+  // correction is subtracted from the first SNP column
+  // (src/Algorithm.cpp:1390-1396).
+  double X[LB][R];
+  double Clr[R > 1 ? LB : 1][R];
+  double Crr[RR];
+#pragma unroll
+  for (int j = 0; j < LB; ++j) {
+#pragma unroll
+    for (int c = 0; c < R; ++c) {
+      X[j][c] = (j < l) ? d.Str[((size_t)s * l + j) * R + c] : 0.0;
+      if (R > 1) Clr[R > 1 ? j : 0][c] = 0.0;
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < RR; ++e) Crr[e] = 0.0;
+  if (cnt > 0) {
+    const size_t base = (size_t)s * d.Q;
+#pragma unroll
+    for (int j = 0; j < LB; ++j) {
+      if (j < l) {
+        double tot = 0.0;
+#pragma unroll
+        for (int c = 0; c < R; ++c) {
+          const double v = d.Ccorr[(base + j * R + c) * G + g];
+          tot += v;
+          if (R > 1) Clr[R > 1 ? j : 0][c] = v;
+        }
+        X[j][0] -= tot;
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < RR; ++e) Crr[e] = d.Ccorr[(base + l * R + e) * G + g];
+  }
+  // D_q: Sbr corrected only in rows j < min(l, r) (src/Algorithm.cpp:1300).
+  double Sc[RR];
+#pragma unroll
+  for (int b2 = 0; b2 < R; ++b2)
+#pragma unroll
+    for (int a = 0; a <= b2; ++a)
+      Sc[upk(a, b2)] = d.Sbr[(size_t)s * RR + upk(a, b2)] - ((a < l) ? Crr[upk(a, b2)] : 0.0);
+
+  // X = U_A^-T B_q
+#pragma unroll
+  for (int j = 0; j < LB; ++j) {
+    if (j < l) {
+      const double inv = d.UA_invd[(size_t)j * G + g];
+#pragma unroll
+      for (int c = 0; c < R; ++c) {
+        double v = X[j][c];
+#pragma unroll
+        for (int i = 0; i < j; ++i) v -= d.UA[(size_t)upk(i, j) * G + g] * X[i][c];
+        X[j][c] = v * inv;
+      }
+    }
+  }
+  // Schur complement and its Cholesky factor.
+#pragma unroll
+  for (int b2 = 0; b2 < R; ++b2)
+#pragma unroll
+    for (int a = 0; a <= b2; ++a) {
+      double v = Sc[upk(a, b2)];
+#pragma unroll
+      for (int j = 0; j < LB; ++j) v -= X[j][a] * X[j][b2];
+      Sc[upk(a, b2)] = v;
+    }
+  bool ok = d.pat_pd[g] != 0;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    double ajj = Sc[upk(j, j)];
+#pragma unroll
+    for (int k = 0; k < j; ++k) ajj -= Sc[upk(k, j)] * Sc[upk(k, j)];
+    if (!(ajj > 0.0)) ok = false;
+    ajj = sqrt(ajj);
+    Sc[upk(j, j)] = ajj;
+#pragma unroll
+    for (int i = j + 1; i < R; ++i) {
+      double aji = Sc[upk(j, i)];
+#pragma unroll
+      for (int k = 0; k < j; ++k) aji -= Sc[upk(k, j)] * Sc[upk(k, i)];
+      Sc[upk(j, i)] = aji / ajj;
+    }
+  }
+  double zt[LB];
+#pragma unroll
+  for (int j = 0; j < LB; ++j) zt[j] = (j < l) ? d.zt[(size_t)j * d.ldt + y] : 0.0;
+
+  double b[P];
+#pragma unroll
+  for (int h = 0; h < P; ++h) b[h] = 0.0;
+  double rss = 0.0;
+  if (ok) {
+    double z[R];
+#pragma unroll
+    for (int c = 0; c < R; ++c) {
+      double v = rb[c];
+#pragma unroll
+      for (int j = 0; j < LB; ++j) v -= X[j][c] * zt[j];
+      z[c] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < R; ++i) {   // U_S' z = num
+      double v = z[i];
+#pragma unroll
+      for (int k = 0; k < i; ++k) v -= Sc[upk(k, i)] * z[k];
+      z[i] = v / Sc[upk(i, i)];
+    }
+    double zz = 0.0;
+#pragma unroll
+    for (int c = 0; c < R; ++c) zz += z[c] * z[c];
+    double br[R];
+#pragma unroll
+    for (int i = R - 1; i >= 0; --i) {   // U_S b_r = z
+      double v = z[i];
+#pragma unroll
+      for (int k = i + 1; k < R; ++k) v -= Sc[upk(i, k)] * br[k];
+      br[i] = v / Sc[upk(i, i)];
+    }
+    rss = d.rss0[y] - zz;
+#pragma unroll
+    for (int c = 0; c < R; ++c) b[LB + c] = br[c];
+    const bool quirk = (R > 1) && cnt > 0;
+    const bool need_bl = d.need_bl || quirk || d.yzero[y] > 0 || !(rss >= RSS_GUARD * d.yy[y]);
+    if (need_bl) {
+      // b_l = U_A^-1 (zt - X b_r)
+#pragma unroll
+      for (int j = LB - 1; j >= 0; --j) {
+        if (j < l) {
+          double v = zt[j];
+#pragma unroll
+          for (int c = 0; c < R; ++c) v -= X[j][c] * br[c];
+#pragma unroll
+          for (int i = j + 1; i < LB; ++i)
+            if (i < l) v -= d.UA[(size_t)upk(j, i) * G + g] * b[i];
+          b[j] = v * d.UA_invd[(size_t)j * G + g];
+        }
+      }
+    }
+    if (quirk) {
+      // RSS of the reference's b on the true masked Gram matrix: add
+      // b' (S_true - S_q) b, nonzero in the B block (first SNP column got
+      // every correction) and in D rows >= l (never corrected).
+      double q = 0.0;
+#pragma unroll
+      for (int j = 0; j < LB; ++j) {
+        if (j < l) {
+          double rest = 0.0;
+#pragma unroll
+          for (int c = 1; c < R; ++c) rest += Clr[R > 1 ? j : 0][c];
+          double v = rest * br[0];
+#pragma unroll
+          for (int c = 1; c < R; ++c) v -= Clr[R > 1 ? j : 0][c] * br[c];
+          q += 2.0 * b[j] * v;
+        }
+      }
+#pragma unroll
+      for (int b2 = 0; b2 < R; ++b2)
+#pragma unroll
+        for (int a = 0; a <= b2; ++a)
+          if (a >= l) q -= (a == b2 ? 1.0 : 2.0) * br[a] * Crr[upk(a, b2)] * br[b2];
+      rss += q;
+    }
+  }
+  // Compact coefficient vector: b[0..l) covariates, b[l..l+R) SNP terms.
+  double bc[P];
+#pragma unroll
+  for (int h = 0; h < P; ++h) bc[h] = 0.0;
+#pragma unroll
+  for (int h = 0; h < LB; ++h)
+    if (h < l) bc[h] = b[h];
+#pragma unroll
+  for (int c = 0; c < R; ++c) {
+#pragma unroll
+    for (int h = 0; h < P; ++h)
+      if (h == l + c) bc[h] = b[LB + c];
+  }
+  const bool fallback = !ok || d.yzero[y] > 0 || !(rss >= RSS_GUARD * d.yy[y]);
+  if (fallback) {
+    if (!ok) {   // sposv left the right-hand side in place
+#pragma unroll
+      for (int h = 0; h < P; ++h) {
+        if (h < l) bc[h] = d.rt[(size_t)h * d.ldt + y];
+      }
+#pragma unroll
+      for (int c = 0; c < R; ++c)
+#pragma unroll
+        for (int h = 0; h < P; ++h)
+          if (h == l + c) bc[h] = rb[c];
+    }
+    const unsigned long long slot = atomicAdd(&d.counters[CNT_FB_CHUNK], 1ull);
+    atomicAdd(&d.counters[CNT_FALLBACK], 1ull);
+    if (slot < (unsigned long long)d.fb_cap) {
+      FallbackEntry* f = d.fb + slot;
+      f->y = y;
+      f->s = s;
+      f->flags = ok ? 0 : 1;
+#pragma unroll
+      for (int h = 0; h < P; ++h) f->b[h] = bc[h];
+      return;
+    }
+    rss = explicit_rss_thread<R, LB>(d, y, s, bc);
+  }
+  stats_emit<P>(d, y, s, bc, rss);
+}
+
+// Explicit residual pass for the pairs the epilogue handed over: one wave
+// per pair.
+template <int R, int LB>
+__global__ __launch_bounds__(256) void k_fallback(DevScan d) {
+  constexpr int P = LB + R;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  unsigned long long total = d.counters[CNT_FB_CHUNK];
+  const int count = (int)(total < (unsigned long long)d.fb_cap ? total : (unsigned long long)d.fb_cap);
+  for (int e = blockIdx.x * 4 + w; e < count; e += gridDim.x * 4) {
+    const FallbackEntry* f = d.fb + e;
+    const int y = f->y, s = f->s;
+    double b[P];
+#pragma unroll
+    for (int h = 0; h < P; ++h) b[h] = f->b[h];
+    int src[R], env[R];
+    col_desc<R>(d, d.s0 + s, src, env);
+    double mean[R];
+#pragma unroll
+    for (int c = 0; c < R; ++c) mean[c] = d.col_mean[(size_t)s * R + c];
+    double acc = 0.0;
+    for (int k = lane; k < d.n; k += 64) {
+      double yv = load_elem(d.Y, d.ydt, (size_t)y * d.ldy + k);
+      if (yv != yv) yv = 0.0;
+      if (yv == 0.0) continue;
+      double sy = 0.0;
+#pragma unroll
+      for (int h = 0; h < LB; ++h)
+        if (h < d.l) sy += d.XL_cm[(size_t)h * d.n_pad + k] * b[h];
+#pragma unroll
+      for (int c = 0; c < R; ++c) {
+        double v = geno_value(d, src[c], env[c], k);
+        if (v != v) v = mean[c];
+#pragma unroll
+        for (int h = 0; h < P; ++h)
+          if (h == d.l + c) sy += v * b[h];
+      }
+      sy -= yv;
+      acc += sy * sy;
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) stats_emit<P>(d, y, s, b, acc);
+  }
+}
+
+__global__ void k_reset_counter(unsigned long long* c, int idx) { c[idx] = 0ull; }
+
+// ---------------------------------------------------------------------------
+// launchers
+template <int R>
+static void prep_r(const DevScan& d, int prec, void* Ahi, void* Alo, hipStream_t st) {
+  dim3 grid(d.mc), block(256);
+  if (prec == PREC_FP64) hipLaunchKernelGGL((k_prep<R, PREC_FP64>), grid, block, 0, st, d, Ahi, Alo);
+  else if (prec == PREC_FP32) hipLaunchKernelGGL((k_prep<R, PREC_FP32>), grid, block, 0, st, d, Ahi, Alo);
+  else hipLaunchKernelGGL((k_prep<R, PREC_BF16X3>), grid, block, 0, st, d, Ahi, Alo);
+}
+
+void launch_prep(const DevScan& d, int prec, void* Ahi, void* Alo, hipStream_t st) {
+  switch (d.r) {
+    case 1: prep_r<1>(d, prec, Ahi, Alo, st); break;
+    case 2: prep_r<2>(d, prec, Ahi, Alo, st); break;
+    case 3: prep_r<3>(d, prec, Ahi, Alo, st); break;
+    default: prep_r<4>(d, prec, Ahi, Alo, st); break;
+  }
+}
+
+// (R, LB) pairs with compiled kernels: R=1: LB 8/16/32; R=2: 8/16; R=3,4: 8.
+#define OA_DISPATCH_RLB(R_, LB_, FN)          \
+  if (d.r == 1 && lb == 8) { FN(1, 8); }      \
+  else if (d.r == 1 && lb == 16) { FN(1, 16); } \
+  else if (d.r == 1) { FN(1, 32); }           \
+  else if (d.r == 2 && lb == 8) { FN(2, 8); } \
+  else if (d.r == 2) { FN(2, 16); }           \
+  else if (d.r == 3) { FN(3, 8); }            \
+  else { FN(4, 8); }
+
+void launch_snp_gram(const DevScan& d, int prec, int lb, hipStream_t st) {
+  dim3 grid((d.mc + 63) / 64), block(64);
+#define OA_SG(R_, LB_)                                                                     \
+  if (prec == PREC_FP64) hipLaunchKernelGGL((k_snp_gram<R_, LB_, double>), grid, block, 0, st, d); \
+  else hipLaunchKernelGGL((k_snp_gram<R_, LB_, float>), grid, block, 0, st, d);
+  OA_DISPATCH_RLB(0, 0, OA_SG)
+#undef OA_SG
+}
+
+void launch_mask(const DevScan& d, int lb, hipStream_t st) {
+  dim3 grid((d.mc + 63) / 64, (d.G + 3) / 4), block(256);
+#define OA_MK(R_, LB_) hipLaunchKernelGGL((k_mask<R_, LB_>), grid, block, 0, st, d);
+  OA_DISPATCH_RLB(0, 0, OA_MK)
+#undef OA_MK
+}
+
+void launch_epilogue(const DevScan& d, int prec, int lb, hipStream_t st) {
+  dim3 grid((d.t + 63) / 64, (d.mc + 3) / 4), block(256);
+#define OA_EP(R_, LB_)                                                                          \
+  if (prec == PREC_FP64) hipLaunchKernelGGL((k_epilogue<R_, LB_, double>), grid, block, 0, st, d); \
+  else hipLaunchKernelGGL((k_epilogue<R_, LB_, float>), grid, block, 0, st, d);
+  OA_DISPATCH_RLB(0, 0, OA_EP)
+#undef OA_EP
+}
+
+void launch_fallback(const DevScan& d, int lb, hipStream_t st) {
+  dim3 grid(1024), block(256);
+#define OA_FB(R_, LB_) hipLaunchKernelGGL((k_fallback<R_, LB_>), grid, block, 0, st, d);
+  OA_DISPATCH_RLB(0, 0, OA_FB)
+#undef OA_FB
+}
+
+void launch_reset_counter(unsigned long long* c, int idx, hipStream_t st) {
+  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, st, c, idx);
+}
+
+}  // namespace oa

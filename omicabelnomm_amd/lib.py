@@ -1,0 +1,109 @@
+"""ctypes binding of liboagpu.so (include/oagpu.h).
+
+The library is the product: there is no CPU fallback.  Loading fails loudly
+when liboagpu.so has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboagpu.so")
+
+OK = 0
+ERR_ARG, ERR_HIP, ERR_STATE, ERR_HIT_OVERFLOW = -1, -2, -3, -4
+ERR_NUMERIC, ERR_UNSUPPORTED, ERR_COVARIATES, ERR_NO_DEVICE = -5, -6, -7, -8
+
+FP64, FP32, BF16X3 = 0, 1, 2
+F64, F32, F16 = 0, 1, 2
+HOST, DEVICE = 0, 1
+
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3}
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int), ("l", ctypes.c_int), ("r", ctypes.c_int),
+        ("real_n", ctypes.c_int), ("precision", ctypes.c_int),
+        ("disp_cov", ctypes.c_int), ("store_all", ctypes.c_int),
+        ("min_r2_store", ctypes.c_float), ("min_p_store", ctypes.c_float),
+        ("min_p_disp", ctypes.c_float), ("max_traits", ctypes.c_int),
+        ("max_snps", ctypes.c_int), ("max_hits", ctypes.c_longlong),
+        ("device", ctypes.c_int),
+    ]
+
+
+class Hit(ctypes.Structure):
+    _fields_ = [
+        ("trait", ctypes.c_int32), ("snp", ctypes.c_int32), ("h", ctypes.c_int32),
+        ("n_used", ctypes.c_int32), ("b", ctypes.c_double), ("se", ctypes.c_double),
+        ("t", ctypes.c_double), ("p", ctypes.c_double), ("r2", ctypes.c_double),
+    ]
+
+
+HIT_DTYPE = np.dtype([("trait", "<i4"), ("snp", "<i4"), ("h", "<i4"), ("n_used", "<i4"),
+                      ("b", "<f8"), ("se", "<f8"), ("t", "<f8"), ("p", "<f8"), ("r2", "<f8")])
+assert HIT_DTYPE.itemsize == ctypes.sizeof(Hit)
+
+
+class Dense(ctypes.Structure):
+    _fields_ = [
+        ("beta", ctypes.c_void_p), ("se", ctypes.c_void_p), ("tstat", ctypes.c_void_p),
+        ("pval", ctypes.c_void_p), ("kept", ctypes.c_void_p), ("r2", ctypes.c_void_p),
+        ("rss", ctypes.c_void_p), ("emitted", ctypes.c_void_p), ("n_used", ctypes.c_void_p),
+        ("var_x", ctypes.c_void_p),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("n_hits", ctypes.c_longlong), ("n_emitted", ctypes.c_longlong),
+        ("n_sig", ctypes.c_longlong), ("n_fallback", ctypes.c_longlong),
+        ("n_patterns", ctypes.c_int), ("n_chunks", ctypes.c_int), ("t_star", ctypes.c_double),
+    ]
+
+
+EXPORTS = ["oagpu_create", "oagpu_destroy", "oagpu_last_error", "oagpu_set_covariates",
+           "oagpu_set_interactions", "oagpu_load_traits", "oagpu_load_genotypes", "oagpu_scan",
+           "oagpu_last_timings", "oagpu_tvalue"]
+
+_lib = None
+
+
+def load():
+    """Load liboagpu.so and declare its signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("liboagpu.so is not built (%s); run omicabelnomm_amd.build.build()" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    vp, ll, i = ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int
+    L.oagpu_create.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(vp)]
+    L.oagpu_create.restype = i
+    L.oagpu_destroy.argtypes = [vp]
+    L.oagpu_destroy.restype = None
+    L.oagpu_last_error.argtypes = [vp]
+    L.oagpu_last_error.restype = ctypes.c_char_p
+    L.oagpu_set_covariates.argtypes = [vp, vp, ll]
+    L.oagpu_set_interactions.argtypes = [vp, vp, ll, i]
+    L.oagpu_load_traits.argtypes = [vp, vp, i, i, ll, i, i]
+    L.oagpu_load_genotypes.argtypes = [vp, vp, i, i, ll, i, i, vp, vp, i]
+    L.oagpu_scan.argtypes = [vp, ctypes.POINTER(Dense), vp, ll, ctypes.POINTER(Stats)]
+    L.oagpu_last_timings.argtypes = [vp, ctypes.POINTER(ctypes.c_double), i]
+    L.oagpu_tvalue.argtypes = [ctypes.c_float]
+    L.oagpu_tvalue.restype = ctypes.c_double
+    for f in ["oagpu_set_covariates", "oagpu_set_interactions", "oagpu_load_traits",
+              "oagpu_load_genotypes", "oagpu_scan", "oagpu_last_timings"]:
+        getattr(L, f).restype = i
+    _lib = L
+    return L
+
+
+class OagpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("oagpu error %d: %s" % (code, msg))
+        self.code = code

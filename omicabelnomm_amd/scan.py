@@ -1,0 +1,242 @@
+"""Host-side mirror of the reference's scan interface over liboagpu.so.
+
+`Settings`, `Outputs`, `Algorithm.applyDefaultParams` and the threshold
+normalisation follow src/Definitions.h:168-257 and src/Algorithm.cpp:277-360.
+`Scanner` exposes the in-memory block interface of the C ABI
+(include/oagpu.h): covariates, a trait block, a genotype block, scan.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import lib as L
+
+P_NEQ_B_OPT_MD = 7  # the only solver id the reference implements (src/Algorithm.h:29)
+
+
+@dataclass
+class Settings:
+    """Subset of the reference Settings struct (src/Definitions.h:168-219)
+    that the scan reads, plus the precision/device of this implementation."""
+    n: int = 0
+    t: int = 0
+    m: int = 0
+    l: int = 0
+    r: int = 1
+    p: int = 0
+    tb: int = 0
+    mb: int = 0
+    threads: int = 1
+    storeBin: bool = False
+    minPdisp: float = 0.05
+    minPstore: float = 0.1
+    minR2disp: float = 0.000001
+    minR2store: float = 0.00001
+    storePInd: bool = False
+    disp_cov: bool = False
+    fnameAL: str = ""
+    fnameAR: str = ""
+    fnameY: str = ""
+    fnameOutFiles: str = ""
+    fname_excludelist: str = ""
+    use_interactions: bool = False
+    keep_depVar: bool = False
+    use_multiple_interaction_sets: bool = False
+    fname_interactions: str = ""
+    ini_IDX_interactions: int = -1
+    end_IDX_interactions: int = -1
+    limit_m: int = 2**31 - 1
+    limit_t: int = 2**31 - 1
+    limit_n: int = 2**31 - 1
+    precision: str = "fp64"
+    device: int = 0
+
+
+@dataclass
+class Outputs:
+    """Reference Outputs (src/Definitions.h:222-257): the fields this
+    implementation fills."""
+    duration: float = 0.0
+    total_sig_results: int = 0
+    n_hits: int = 0
+    n_emitted: int = 0
+    n_fallback: int = 0
+    timings_ms: dict = field(default_factory=dict)
+
+
+class Algorithm:
+    def applyDefaultParams(self, params: Settings) -> None:
+        """src/Algorithm.cpp:277-311."""
+        d = Settings()
+        for k in ("fname_excludelist", "disp_cov", "storePInd", "storeBin", "threads", "r",
+                  "use_interactions", "keep_depVar", "use_multiple_interaction_sets",
+                  "fname_interactions", "ini_IDX_interactions", "end_IDX_interactions",
+                  "minR2store", "minR2disp", "minPstore", "minPdisp", "limit_m", "limit_t", "limit_n"):
+            setattr(params, k, getattr(d, k))
+
+
+def normalize_thresholds(params: Settings) -> Settings:
+    """Threshold normalisation at the top of partialNEQ_Blocked_STL_MD
+    (src/Algorithm.cpp:338-342): with -b, or a display threshold looser
+    than the storage one, storage uses the display threshold."""
+    if params.minR2disp < params.minR2store or params.storeBin:
+        params.minR2store = params.minR2disp
+    if params.minPdisp > params.minPstore or params.storeBin:
+        params.minPstore = params.minPdisp
+    return params
+
+
+def tvalue(pval: float) -> float:
+    """getTvalue (src/Utility.cpp:244-250)."""
+    return L.load().oagpu_tvalue(float(np.float32(pval)))
+
+
+def _device_ptr(x):
+    """(pointer, where, dtype code, ld, ncols) for a numpy or torch array.
+
+    numpy arrays are (rows, cols) matrices stored column-major (Fortran) or
+    made so; torch CUDA tensors are given as (cols, ld) row-major, i.e. one
+    row per column of the matrix."""
+    try:
+        import torch  # noqa: F401
+        is_torch = type(x).__module__.startswith("torch")
+    except ImportError:  # pragma: no cover
+        is_torch = False
+    if is_torch:
+        if not x.is_cuda:
+            x = x.numpy()
+        else:
+            if x.dim() != 2 or x.stride(1) != 1:
+                raise ValueError("device arrays must be (columns, ld) row-major tensors")
+            dt = {"torch.float64": L.F64, "torch.float32": L.F32, "torch.float16": L.F16}[str(x.dtype)]
+            return x.data_ptr(), L.DEVICE, dt, x.stride(0), x.shape[0], x
+    a = np.asarray(x)
+    if a.dtype not in (np.float64, np.float32, np.float16):
+        a = a.astype(np.float64)
+    a = np.asfortranarray(a)
+    dt = {np.dtype(np.float64): L.F64, np.dtype(np.float32): L.F32, np.dtype(np.float16): L.F16}[a.dtype]
+    return a.ctypes.data, L.HOST, dt, a.shape[0], a.shape[1], a
+
+
+@dataclass
+class ScanResult:
+    hits: np.ndarray
+    stats: dict
+    dense: Optional[dict]
+    timings_ms: dict
+
+
+class Scanner:
+    """One liboagpu context: covariates, then trait and genotype blocks."""
+
+    def __init__(self, n, l, r=1, precision="fp64", *, real_n=None, disp_cov=False,
+                 store_all=False, min_p_store=0.1, min_p_disp=0.05, min_r2_store=1e-5,
+                 max_traits=1024, max_snps=0, max_hits=1 << 22, device=0):
+        self._L = L.load()
+        self.n, self.l, self.r, self.p = n, l, r, l + r
+        self.params = L.Params(n=n, l=l, r=r, real_n=real_n or n,
+                               precision=L.PRECISIONS[precision] if isinstance(precision, str) else precision,
+                               disp_cov=int(disp_cov), store_all=int(store_all),
+                               min_r2_store=min_r2_store, min_p_store=min_p_store,
+                               min_p_disp=min_p_disp, max_traits=max_traits, max_snps=max_snps,
+                               max_hits=max_hits, device=device)
+        self._ctx = ctypes.c_void_p()
+        rc = self._L.oagpu_create(ctypes.byref(self.params), ctypes.byref(self._ctx))
+        self._check(rc)
+        self._keep = {}
+        self.t = 0
+        self.m = 0
+
+    def _check(self, rc):
+        if rc != L.OK:
+            msg = self._L.oagpu_last_error(self._ctx).decode() if self._ctx else "create failed"
+            raise L.OagpuError(rc, msg)
+
+    def close(self):
+        if self._ctx:
+            self._L.oagpu_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_covariates(self, XL):
+        XL = np.asfortranarray(XL, dtype=np.float64)
+        assert XL.shape == (self.n, self.l)
+        self._check(self._L.oagpu_set_covariates(self._ctx, XL.ctypes.data, XL.shape[0]))
+
+    def set_interactions(self, E):
+        E = np.asfortranarray(E, dtype=np.float64)
+        assert E.shape[0] == self.n
+        self._check(self._L.oagpu_set_interactions(self._ctx, E.ctypes.data, E.shape[0], E.shape[1]))
+
+    def load_traits(self, Y, trait_offset=0):
+        ptr, where, dt, ld, t, keep = _device_ptr(Y)
+        self._keep["Y"] = keep
+        self._check(self._L.oagpu_load_traits(self._ctx, ptr, dt, where, ld, t, trait_offset))
+        self.t = t
+
+    def load_genotypes(self, G, m=None, col_src=None, col_env=None, snp_offset=0):
+        ptr, where, dt, ld, n_src, keep = _device_ptr(G)
+        self._keep["G"] = keep
+        if m is None:
+            m = n_src // self.r if col_src is None else len(col_src) // self.r
+        src = env = None
+        if col_src is not None:
+            src = np.ascontiguousarray(col_src, dtype=np.int32)
+            assert src.size == m * self.r
+        if col_env is not None:
+            env = np.ascontiguousarray(col_env, dtype=np.int32)
+            assert env.size == m * self.r
+        self._keep["desc"] = (src, env)
+        self._check(self._L.oagpu_load_genotypes(
+            self._ctx, ptr, dt, where, ld, n_src, m,
+            src.ctypes.data if src is not None else None,
+            env.ctypes.data if env is not None else None, snp_offset))
+        self.m = m
+
+    def scan(self, dense=False, hit_cap=None) -> ScanResult:
+        t, m, p = self.t, self.m, self.p
+        dres = None
+        dstruct = L.Dense()
+        if dense:
+            dres = {
+                "beta": np.zeros((t, m, p)), "se": np.zeros((t, m, p)),
+                "tstat": np.zeros((t, m, p)), "pval": np.zeros((t, m, p)),
+                "kept": np.zeros((t, m, p), dtype=np.uint8),
+                "r2": np.zeros((t, m)), "rss": np.zeros((t, m)),
+                "emitted": np.zeros((t, m), dtype=np.uint8),
+                "n_used": np.zeros(t, dtype=np.int32), "var_x": np.zeros((m, p)),
+            }
+            for k, v in dres.items():
+                setattr(dstruct, k, v.ctypes.data)
+        cap = int(hit_cap if hit_cap is not None else self.params.max_hits)
+        hits = np.zeros(cap, dtype=L.HIT_DTYPE)
+        st = L.Stats()
+        rc = self._L.oagpu_scan(self._ctx, ctypes.byref(dstruct) if dense else None,
+                                hits.ctypes.data, cap, ctypes.byref(st))
+        stats = {k: getattr(st, k) for k, _ in L.Stats._fields_}
+        if rc == L.ERR_HIT_OVERFLOW:
+            raise L.OagpuError(rc, self._L.oagpu_last_error(self._ctx).decode() + " stats=%s" % stats)
+        self._check(rc)
+        return ScanResult(hits=hits[: stats["n_hits"]].copy(), stats=stats, dense=dres,
+                          timings_ms=self.timings())
+
+    def timings(self) -> dict:
+        buf = (ctypes.c_double * 7)()
+        k = self._L.oagpu_last_timings(self._ctx, buf, 7)
+        names = ["prep", "gemm", "snp_gram", "corrections", "epilogue", "fallback", "total"]
+        return {names[i]: buf[i] for i in range(k)}
