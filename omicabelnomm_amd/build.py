@@ -17,10 +17,12 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "build")
 LIB = os.path.join(HERE, "liboagpu.so")
+HOSTLIB = os.path.join(HERE, "libomicabelnomm.so")
 CLI = os.path.join(HERE, "omicabelnomm")
 
 HIP_SOURCES = ["oagpu_api.hip", "oagpu_kernels_trait.hip", "oagpu_kernels_snp.hip", "oagpu_gemm.hip"]
-HOST_SOURCES = ["host/main.cpp", "host/databel.cpp", "host/writers.cpp", "host/driver.cpp"]
+HOST_SOURCES = ["host/databel.cpp", "host/writers.cpp", "host/driver.cpp"]
+CLI_SOURCES = ["host/main.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
@@ -56,18 +58,27 @@ def build(verbose: bool = True, cli: bool = True) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
-    if cli and all(os.path.exists(os.path.join(CSRC, s)) for s in HOST_SOURCES):
-        hobjs = []
-        for s in HOST_SOURCES:
-            o = os.path.join(OBJ, "host_" + os.path.basename(s).replace(".cpp", ".o"))
-            _run(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-I" + os.path.join(ROOT, "include"),
-                  "-c", os.path.join(CSRC, s), "-o", o])
-            hobjs.append(o)
+    if cli:
+        inc = "-I" + os.path.join(ROOT, "include")
+
+        def host_obj(src):
+            o = os.path.join(OBJ, "host_" + os.path.basename(src).replace(".cpp", ".o"))
+            _run(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter", inc,
+                  "-c", os.path.join(CSRC, src), "-o", o])
+            return o
+
+        with ThreadPoolExecutor(max_workers=jobs) as ex:
+            hobjs = list(ex.map(host_obj, HOST_SOURCES + CLI_SOURCES))
+        tmp = HOSTLIB + ".tmp"
+        _run(["g++", "-shared", "-o", tmp] + hobjs[:len(HOST_SOURCES)] +
+             ["-L" + HERE, "-loagpu", "-Wl,-rpath,$ORIGIN", "-lpthread"])
+        os.replace(tmp, HOSTLIB)
         tmpc = CLI + ".tmp"
-        _run(["g++", "-o", tmpc] + hobjs + ["-L" + HERE, "-loagpu", "-Wl,-rpath,$ORIGIN", "-lpthread"])
+        _run(["g++", "-o", tmpc] + hobjs[len(HOST_SOURCES):] +
+             ["-L" + HERE, "-lomicabelnomm", "-loagpu", "-Wl,-rpath,$ORIGIN", "-lpthread"])
         os.replace(tmpc, CLI)
         if verbose:
-            print("built", CLI)
+            print("built", HOSTLIB, "and", CLI)
     return LIB
 
 

@@ -1,0 +1,493 @@
+// Drop-in scan driver: the file-level path of partialNEQ_Blocked_STL_MD
+// (src/Algorithm.cpp:322-886) and AIOwrapper (src/AIOwrapper.cpp) on top of
+// liboagpu.  Inputs: DatABEL files; outputs: <out>_dis.txt and, with -b,
+// <out>_sto.ibin / <out>_sto.dbin.  The scan runs on the GPU; a reader
+// thread prefetches the next genotype block while the current one is
+// scanned (the reference's async_io pthread, src/AIOwrapper.cpp:895-1437).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <functional>
+#include <future>
+#include <iostream>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "databel.h"
+#include "oagpu.h"
+#include "omicabelnomm.h"
+#include "writers.h"
+
+using namespace oahost;
+
+namespace {
+
+struct Fail : std::runtime_error {
+  int code;
+  Fail(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+std::string str_or(const char* p) { return p ? std::string(p) : std::string(); }
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// read_excludeList + splitpair (src/AIOwrapper.cpp:1994-2233): removes rows
+// from keep[]; returns the number of rows removed.
+int apply_exclusion_file(const std::string& fname, int n, std::vector<uint8_t>& keep, bool verbose) {
+  std::ifstream in(fname);
+  if (!in) throw Fail(1, "Error reading exclude list file.");
+  std::string line;
+  int first = 0, second = 0, prev_second = -1, excluded = 0;
+  if (verbose) std::cout << "Excluding Ids: \n";
+  while (std::getline(in, line) && second < n) {
+    std::istringstream iss(line);
+    iss >> first;
+    iss >> second;
+    if (prev_second > first) {
+      std::ostringstream m;
+      m << "\nPlease provide valid positive ordered indixes!\n" << prev_second << " > " << first << "\n";
+      throw Fail(1, m.str());
+    }
+    if (second == prev_second) {
+      if (verbose) std::cout << first << ", ";
+      second = first;
+    } else {
+      if (verbose) std::cout << first << "-" << second << ", ";
+      if (first > second) throw Fail(1, "\nPlease provide ordered pairs!\n");
+      if (first < 1 || second < 1) throw Fail(1, "\nPlease provide valid positive indixes!\n");
+    }
+    prev_second = second;
+    for (int i = first - 1; i <= second - 1; ++i)
+      if (i >= 0 && i < n && keep[i]) {
+        keep[i] = 0;
+        ++excluded;
+      }
+  }
+  if (excluded >= n) {
+    std::ostringstream m;
+    m << "\nExclusion List excluded all data!\nTotal Ids: " << n << "\nExcluded Ids: " << excluded;
+    throw Fail(1, m.str());
+  }
+  if (verbose) std::cout << "Excluded: " << excluded << " Using: " << n - excluded << std::endl;
+  return excluded;
+}
+
+struct Plan {
+  int n_file = 0, n = 0, realN = 0, l = 0, r_file = 1, r = 1, t = 0, m_file = 0, m = 0;
+  int tb = 0, mb = 0;                  // reference block sizes (output grouping)
+  bool inter = false, multi = false, keep = false;
+  int num_inter = 0, ini_inter = 0, file_block = 0;   // multi-set: file SNPs per block
+  std::vector<uint8_t> keep_rows;
+  std::vector<std::string> y_names, al_names, ar_names;
+};
+
+// Virtual SNP layout: for SNP v, output column c reads genotype file column
+// src(v, c) times interaction column env(v, c).  Columns are relative to
+// the genotype block [f0, f0 + nf) of file SNPs.
+void descriptors(const Plan& P, int f0, int nf, std::vector<int32_t>& src, std::vector<int32_t>& env,
+                 int& v0, int& nv) {
+  src.clear();
+  env.clear();
+  if (!P.inter) {
+    v0 = f0;
+    nv = nf;
+    for (int s = 0; s < nf; ++s)
+      for (int c = 0; c < P.r; ++c) {
+        src.push_back(s * P.r_file + c);
+        env.push_back(-1);
+      }
+    return;
+  }
+  if (!P.multi) {   // [I_1*X, ..., I_c*X, (X)] per SNP (src/AIOwrapper.cpp:796-840)
+    v0 = f0;
+    nv = nf;
+    for (int s = 0; s < nf; ++s) {
+      for (int h = 0; h < P.num_inter; ++h) {
+        src.push_back(s);
+        env.push_back(h);
+      }
+      if (P.keep) {
+        src.push_back(s);
+        env.push_back(-1);
+      }
+    }
+    return;
+  }
+  // One analysis per interaction: file blocks of file_block SNPs; inside a
+  // block, interaction h outer and SNP inner (src/AIOwrapper.cpp:767-794).
+  v0 = f0 * P.num_inter;
+  nv = nf * P.num_inter;
+  for (int b0 = 0; b0 < nf; b0 += P.file_block) {
+    const int bs = std::min(P.file_block, nf - b0);
+    for (int h = 0; h < P.num_inter; ++h)
+      for (int i = 0; i < bs; ++i) {
+        src.push_back(b0 + i);
+        env.push_back(h);
+        if (P.keep) {
+          src.push_back(b0 + i);
+          env.push_back(-1);
+        }
+      }
+  }
+}
+
+}  // namespace
+
+extern "C" void omicabelnomm_apply_defaults(omicabelnomm_settings* s) {
+  // src/Algorithm.cpp:277-311
+  std::memset(s, 0, sizeof(*s));
+  s->fname_excludelist = "";
+  s->fname_interactions = "";
+  s->threads = 1;
+  s->r = 1;
+  s->ini_IDX_interactions = -1;
+  s->end_IDX_interactions = -1;
+  s->minR2store = 0.00001f;
+  s->minR2disp = 0.000001f;
+  s->minPstore = 0.1f;
+  s->minPdisp = 0.05f;
+  s->limit_m = 2147483647;
+  s->limit_t = 2147483647;
+  s->limit_n = 2147483647;
+  s->precision = OAGPU_FP64;
+  s->device = 0;
+  s->verbose = 1;
+}
+
+static int solve_impl(const omicabelnomm_settings* sin, omicabelnomm_outputs* out) {
+  omicabelnomm_settings s = *sin;
+  const bool verbose = s.verbose != 0;
+  // Threshold normalisation (src/Algorithm.cpp:338-342).
+  if (s.minR2disp < s.minR2store || s.storeBin) s.minR2store = s.minR2disp;
+  if (s.minPdisp > s.minPstore || s.storeBin) s.minPstore = s.minPdisp;
+
+  const std::string fY = str_or(s.fnameY), fAL = str_or(s.fnameAL), fAR = str_or(s.fnameAR),
+                    fOut = str_or(s.fnameOutFiles);
+  Fvi yfvi = read_fvi(fY + ".fvi");
+  Fvi alfvi = read_fvi(fAL + ".fvi");
+  Fvi arfvi = read_fvi(fAR + ".fvi");
+  if (alfvi.nobs != arfvi.nobs || arfvi.nobs != yfvi.nobs || alfvi.nobs != yfvi.nobs) {
+    std::ostringstream m;
+    m << "The number of elements/individuals from the input files do not coincide with each other! See:\n"
+      << fY << ":" << yfvi.nobs << "\n" << fAL << ":" << alfvi.nobs << "\n" << fAR << ":" << arfvi.nobs;
+    throw Fail(1, m.str());
+  }
+  Plan P;
+  P.r_file = std::max(1, s.r);
+  P.n_file = std::min((int)alfvi.nobs, s.limit_n);
+  P.realN = P.n_file;
+  if ((int)arfvi.nvars % P.r_file != 0) {
+    std::ostringstream m;
+    m << "Number of elements in " << fAR << ".fvi:" << arfvi.nvars
+      << " is not consistent with the value of --ngpred (-n)" << P.r_file << " provided";
+    throw Fail(1, m.str());
+  }
+  P.m_file = std::min((int)arfvi.nvars / P.r_file, s.limit_m / P.r_file);
+  P.t = std::min((int)yfvi.nvars, s.limit_t);
+  P.l = (int)alfvi.nvars;
+  for (int i = 0; i < P.n_file; ++i) {
+    const std::string yn = yfvi.obs_name(i), an = alfvi.obs_name(i), rn = arfvi.obs_name(i);
+    if (yn != an || yn != rn) {
+      if (verbose)
+        std::cout << "Warning, ID names between the -p, -c and -g files do not coincide! The files must "
+                     "have the same meaningful order"
+                  << std::endl;
+      break;
+    }
+  }
+  for (int i = 0; i < P.t; ++i) P.y_names.push_back(yfvi.var_name(i));
+  for (int i = 0; i < P.l; ++i) P.al_names.push_back(alfvi.var_name(i));
+
+  // Row exclusion: exclusion list, then covariate NaN, then interaction NaN
+  // (src/AIOwrapper.cpp:229-253, 434-470).
+  P.keep_rows.assign(P.n_file, 1);
+  const std::string fexcl = str_or(s.fname_excludelist);
+  if (!fexcl.empty()) apply_exclusion_file(fexcl, P.n_file, P.keep_rows, verbose);
+  std::vector<double> xl_all;
+  {
+    FvdReader rd(fAL + ".fvd", alfvi);
+    rd.read_columns(0, P.l, xl_all);
+    int almiss = 0;
+    for (int h = 0; h < P.l; ++h)
+      for (int i = 0; i < P.n_file; ++i)
+        if (std::isnan(xl_all[(size_t)h * alfvi.nobs + i]) && P.keep_rows[i]) {
+          P.keep_rows[i] = 0;
+          ++almiss;
+        }
+    if (almiss && verbose) std::cout << "Excluding " << almiss << " from covariate missings" << std::endl;
+  }
+  std::vector<double> e_all;
+  Fvi ifvi;
+  if (s.use_interactions) {
+    P.inter = true;
+    P.multi = s.use_multiple_interaction_sets != 0;
+    P.keep = s.keep_depVar != 0;
+    if (P.r_file != 1)
+      throw Fail(1, "Cannot use Interactions with --ngpred set to: " + std::to_string(P.r_file) +
+                        ". Required is 1!");
+    const std::string fI = str_or(s.fname_interactions);
+    ifvi = read_fvi(fI + ".fvi");
+    if (ifvi.nobs != arfvi.nobs)
+      throw Fail(1, "The number of elements/individuals from the input files do not coincide with each other!");
+    if (s.end_IDX_interactions != -1 || s.ini_IDX_interactions != -1) {
+      P.num_inter = s.end_IDX_interactions - s.ini_IDX_interactions + 1;
+      P.ini_inter = s.ini_IDX_interactions;
+    } else {
+      P.num_inter = (int)ifvi.nvars;
+      P.ini_inter = 0;
+    }
+    if (P.num_inter <= 0 || P.ini_inter < 0 || P.ini_inter + P.num_inter > (int)ifvi.nvars)
+      throw Fail(1, "The specified range for data to be read from the Interactions file exceeds its content dimensions!");
+    FvdReader rd(fI + ".fvd", ifvi);
+    rd.read_columns(P.ini_inter, P.num_inter, e_all);
+    int imiss = 0;
+    for (int h = 0; h < P.num_inter; ++h)
+      for (int i = 0; i < P.n_file; ++i)
+        if (std::isnan(e_all[(size_t)h * ifvi.nobs + i]) && P.keep_rows[i]) {
+          P.keep_rows[i] = 0;
+          ++imiss;
+        }
+    if (imiss && verbose) std::cout << "Excluding " << imiss << " from Interaction missings" << std::endl;
+  }
+  P.n = 0;
+  for (int i = 0; i < P.n_file; ++i) P.n += P.keep_rows[i];
+
+  // Model shape and AR names (src/AIOwrapper.cpp:421-446, 563-657).
+  const int mb_file = std::min(P.m_file, 500);
+  std::vector<std::string> file_names;
+  for (int i = 0; i < P.m_file * P.r_file; ++i) file_names.push_back(arfvi.var_name(i));
+  if (!P.inter) {
+    P.r = P.r_file;
+    P.m = P.m_file;
+    P.mb = mb_file;
+    P.ar_names = file_names;
+  } else if (!P.multi) {
+    P.r = P.num_inter + (P.keep ? 1 : 0);
+    P.m = P.m_file;
+    P.mb = mb_file;
+    for (int i = 0; i < P.m_file; ++i) {
+      for (int h = 0; h < P.num_inter; ++h)
+        P.ar_names.push_back("I" + std::to_string(h + 1) + "x" + file_names[i]);
+      if (P.keep) P.ar_names.push_back(file_names[i]);
+    }
+  } else {
+    P.r = 1 + (P.keep ? 1 : 0);
+    P.file_block = std::max(1, mb_file / P.num_inter);
+    P.m = P.m_file * P.num_inter;
+    P.mb = P.file_block * P.num_inter;
+    for (int b0 = 0; b0 < P.m_file; b0 += P.file_block) {
+      const int bs = std::min(P.file_block, P.m_file - b0);
+      for (int h = 0; h < P.num_inter; ++h) {
+        const std::string pre = "I" + std::to_string(h + 1) + "_";
+        for (int i = 0; i < bs; ++i) {
+          P.ar_names.push_back(pre + file_names[b0 + i]);
+          if (P.keep) P.ar_names.push_back(pre + "o_" + file_names[b0 + i]);
+        }
+      }
+    }
+  }
+  P.tb = std::min(P.t, 1000);
+  const int p = P.l + P.r;
+  if (p >= P.n)
+    throw Fail(1, "Error! The amount of unknows (individuals|elements) must be greater than total of "
+                  "variables (#covariates+#ind.variables)!");
+  if (P.m <= 0 || P.t <= 0) throw Fail(1, "Nothing to scan: no SNPs or no traits.");
+  if (verbose && (P.l > 255 || p > 255 || P.n > 65535))
+    std::cout << "Warning, output binary format does not yet support current problem sizes for the "
+                 "provided p, l, r and n.\nOmitting outputfile."
+              << std::endl;
+
+  if (s.storeBin) {
+    std::vector<const char*> aln, arn, yn;
+    for (int i = 1; i < P.l; ++i) aln.push_back(alfvi.var_name_raw(i));
+    for (int i = 0; i < P.m_file * P.r_file; ++i) arn.push_back(arfvi.var_name_raw(i));
+    for (int i = 0; i < P.t; ++i) yn.push_back(yfvi.var_name_raw(i));
+    write_ibin(fOut, P.n, P.l, P.r, P.t, P.m, arfvi.namelength, s.disp_cov, s.storePInd, aln, arn, yn);
+  }
+
+  WriterConfig wc;
+  wc.out = fOut;
+  wc.store_bin = s.storeBin;
+  wc.store_all = s.storePInd;
+  wc.disp_cov = s.disp_cov;
+  wc.min_p_disp = s.minPdisp;
+  wc.min_r2_disp = s.minR2disp;
+  wc.l = P.l;
+  wc.r = P.r;
+  wc.real_n = P.realN;
+  wc.y_names = P.y_names;
+  wc.al_names = P.al_names;
+  wc.ar_names = P.ar_names;
+  ResultWriter writer(wc);
+
+  // GPU context.
+  oagpu_params gp{};
+  gp.n = P.n;
+  gp.l = P.l;
+  gp.r = P.r;
+  gp.real_n = P.realN;
+  gp.precision = s.precision;
+  gp.disp_cov = s.disp_cov;
+  gp.store_all = s.storePInd;
+  gp.min_r2_store = s.minR2store;
+  gp.min_p_store = s.minPstore;
+  gp.min_p_disp = s.minPdisp;
+  gp.max_traits = P.tb;
+  gp.max_snps = 0;
+  gp.max_hits = 1ll << 24;
+  gp.device = s.device;
+  oagpu_ctx* ctx = nullptr;
+  int rc = oagpu_create(&gp, &ctx);
+  std::unique_ptr<oagpu_ctx, void (*)(oagpu_ctx*)> guard(ctx, oagpu_destroy);
+  if (rc != OAGPU_OK) throw Fail(rc, ctx ? oagpu_last_error(ctx) : "no GPU device");
+
+  std::vector<double> XL((size_t)P.n * P.l);
+  for (int h = 0; h < P.l; ++h) {
+    size_t o = 0;
+    for (int i = 0; i < P.n_file; ++i)
+      if (P.keep_rows[i]) XL[(size_t)h * P.n + o++] = xl_all[(size_t)h * alfvi.nobs + i];
+  }
+  rc = oagpu_set_covariates(ctx, XL.data(), P.n);
+  if (rc != OAGPU_OK) throw Fail(1, oagpu_last_error(ctx));
+  if (P.inter) {
+    std::vector<double> E((size_t)P.n * P.num_inter);
+    for (int h = 0; h < P.num_inter; ++h) {
+      size_t o = 0;
+      for (int i = 0; i < P.n_file; ++i)
+        if (P.keep_rows[i]) E[(size_t)h * P.n + o++] = e_all[(size_t)h * ifvi.nobs + i];
+    }
+    rc = oagpu_set_interactions(ctx, E.data(), P.n, P.num_inter);
+    if (rc != OAGPU_OK) throw Fail(1, oagpu_last_error(ctx));
+  }
+
+  // Genotype block: a whole number of reference SNP blocks (and of
+  // interaction file blocks), about 512 MiB of host data.
+  const bool g64 = arfvi.type == 6;
+  const size_t gelem = g64 ? 8 : 4;
+  const int unit = P.multi ? P.file_block : mb_file;
+  int gb_file = (int)std::max<size_t>(1, ((512ull << 20) / ((size_t)P.n * gelem * P.r_file)) / unit) * unit;
+  gb_file = std::min(gb_file, ((P.m_file + unit - 1) / unit) * unit);
+  const bool y64 = yfvi.type == 6;
+
+  FvdReader yrd(fY + ".fvd", yfvi);
+  FvdReader grd(fAR + ".fvd", arfvi);
+  std::vector<oagpu_hit> hits(1 << 20);
+  long long total_sig = 0;
+  double acc_loady = 0, acc_loadxr = 0, acc_gpu = 0, acc_store = 0;
+  const double t_start = now_s();
+
+  auto read_geno = [&](int f0, std::vector<unsigned char>& buf) {
+    const int nf = std::min(gb_file, P.m_file - f0);
+    buf.resize((size_t)nf * P.r_file * P.n * gelem);
+    if (g64)
+      grd.read_kept<double>((size_t)f0 * P.r_file, (size_t)nf * P.r_file, P.n_file, P.keep_rows, P.n,
+                            reinterpret_cast<double*>(buf.data()));
+    else
+      grd.read_kept<float>((size_t)f0 * P.r_file, (size_t)nf * P.r_file, P.n_file, P.keep_rows, P.n,
+                           reinterpret_cast<float*>(buf.data()));
+  };
+
+  for (int y0 = 0; y0 < P.t; y0 += P.tb) {
+    const int tcur = std::min(P.tb, P.t - y0);
+    double t0 = now_s();
+    std::vector<unsigned char> ybuf((size_t)tcur * P.n * (y64 ? 8 : 4));
+    if (y64)
+      yrd.read_kept<double>(y0, tcur, P.n_file, P.keep_rows, P.n, reinterpret_cast<double*>(ybuf.data()));
+    else
+      yrd.read_kept<float>(y0, tcur, P.n_file, P.keep_rows, P.n, reinterpret_cast<float*>(ybuf.data()));
+    rc = oagpu_load_traits(ctx, ybuf.data(), y64 ? OAGPU_F64 : OAGPU_F32, OAGPU_HOST, P.n, tcur, y0);
+    if (rc != OAGPU_OK) throw Fail(1, oagpu_last_error(ctx));
+    acc_loady += now_s() - t0;
+
+    // Genotype blocks, the next one read by a helper thread meanwhile.
+    std::vector<unsigned char> gcur, gnext;
+    t0 = now_s();
+    read_geno(0, gcur);
+    acc_loadxr += now_s() - t0;
+    for (int f0 = 0; f0 < P.m_file; f0 += gb_file) {
+      const int nf = std::min(gb_file, P.m_file - f0);
+      std::future<void> pre;
+      if (f0 + gb_file < P.m_file)
+        pre = std::async(std::launch::async, [&, f1 = f0 + gb_file] { read_geno(f1, gnext); });
+      std::vector<int32_t> src, env;
+      int v0 = 0, nv = 0;
+      descriptors(P, 0, nf, src, env, v0, nv);
+      v0 = P.multi ? f0 * P.num_inter : f0;
+      t0 = now_s();
+      rc = oagpu_load_genotypes(ctx, gcur.data(), g64 ? OAGPU_F64 : OAGPU_F32, OAGPU_HOST, P.n,
+                                nf * P.r_file, nv, src.data(), P.inter ? env.data() : nullptr, v0);
+      if (rc != OAGPU_OK) throw Fail(1, oagpu_last_error(ctx));
+      oagpu_stats st{};
+      rc = oagpu_scan(ctx, nullptr, hits.data(), (long long)hits.size(), &st);
+      if (rc == OAGPU_ERR_HIT_OVERFLOW && st.n_hits <= gp.max_hits) {
+        // the device arena held them all: rescan into a host buffer that fits
+        hits.resize((size_t)st.n_hits);
+        rc = oagpu_scan(ctx, nullptr, hits.data(), (long long)hits.size(), &st);
+      }
+      if (rc == OAGPU_ERR_HIT_OVERFLOW)
+        throw Fail(1, std::string(oagpu_last_error(ctx)) + " -- raise the thresholds (-s/-e) or split the input");
+      if (rc != OAGPU_OK) throw Fail(1, oagpu_last_error(ctx));
+      acc_gpu += now_s() - t0;
+      total_sig += st.n_sig;
+      // Regroup into the reference's (trait block, SNP block) lists.
+      t0 = now_s();
+      const size_t nh = (size_t)st.n_hits;
+      std::stable_sort(hits.begin(), hits.begin() + nh, [&](const oagpu_hit& a, const oagpu_hit& b) {
+        return a.snp / P.mb < b.snp / P.mb;
+      });
+      for (size_t i = 0; i < nh;) {
+        const int blk = hits[i].snp / P.mb;
+        size_t j = i;
+        while (j < nh && hits[j].snp / P.mb == blk) ++j;
+        writer.write_block(hits.data() + i, j - i, blk * P.mb);
+        i = j;
+      }
+      acc_store += now_s() - t0;
+      if (pre.valid()) {
+        t0 = now_s();
+        pre.get();
+        acc_loadxr += now_s() - t0;
+        std::swap(gcur, gnext);
+      }
+    }
+  }
+  if (out) {
+    out->duration = now_s() - t_start;
+    out->acc_loady = acc_loady;
+    out->acc_loadxr = acc_loadxr;
+    out->acc_gpu = acc_gpu;
+    out->acc_storeb = acc_store;
+    out->total_sig_results = total_sig;
+    out->n_txt_lines = writer.txt_lines();
+    out->n_bin_results = writer.bin_results();
+    out->n = P.n;
+    out->l = P.l;
+    out->r = P.r;
+    out->t = P.t;
+    out->m = P.m;
+    out->realN = P.realN;
+  }
+  return 0;
+}
+
+extern "C" int omicabelnomm_solve(const omicabelnomm_settings* s, omicabelnomm_outputs* out, char* errbuf,
+                                  int errlen) {
+  if (errbuf && errlen > 0) errbuf[0] = 0;
+  try {
+    return solve_impl(s, out);
+  } catch (const Fail& f) {
+    if (errbuf && errlen > 0) std::snprintf(errbuf, errlen, "%s", f.what());
+    return f.code ? f.code : 1;
+  } catch (const std::exception& e) {
+    if (errbuf && errlen > 0) std::snprintf(errbuf, errlen, "%s", e.what());
+    return 1;
+  }
+}

@@ -20,6 +20,9 @@
 // workgroup ids to the 8 XCDs round-robin, and each XCD receives a
 // contiguous range of tiles (row-tile major), so the tiles that share an A
 // row block run on the same XCD and reuse it from that XCD's L2.
+#include <cstdlib>
+#include <cstring>
+
 #include "oagpu_kernels.h"
 
 namespace oa {
@@ -194,6 +197,113 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf16x3(const uint16_t* __restri
     }
 }
 
+
+// Same contraction with LDS staging: the 128 x 128 workgroup tile's A and B
+// slices (hi and lo) are loaded once per k-step into LDS (double
+// buffered, rows padded to 80 bytes) and every wave reads its MFMA
+// operands from there, so each global byte feeds two waves instead of one.
+constexpr int LDS_ROW = 40;   // bf16 per LDS row: 32 k-values + 8 padding
+
+__global__ __launch_bounds__(256, 1) void k_gemm_bf16x3_lds(const uint16_t* __restrict__ Ah,
+                                                           const uint16_t* __restrict__ Al,
+                                                           const uint16_t* __restrict__ Bh,
+                                                           const uint16_t* __restrict__ Bl,
+                                                           float* __restrict__ C, int M, int N, int K,
+                                                           int ldc, int tiles_n, int total) {
+  __shared__ __attribute__((aligned(16))) uint16_t sm[2][4][128 * LDS_ROW];   // [stage][Ah,Al,Bh,Bl]
+  const int tile = xcd_tile(blockIdx.x, total);
+  const int tm = tile / tiles_n, tn = tile - tm * tiles_n;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
+  const int i = lane & 15, g = lane >> 4;
+  const int wr = (w & 1) * 64, wc = (w >> 1) * 64;
+
+  // global -> register staging: chunk c in {tid, tid + 256}: row c >> 2, 8 k-values at 8*(c & 3)
+  size_t goff[2][2];   // [chunk][A/B]
+  int soff[2];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int c = tid + q * 256, row = c >> 2, ch = c & 3;
+    goff[q][0] = (size_t)(tm * 128 + row) * K + 8 * ch;
+    goff[q][1] = (size_t)(tn * 128 + row) * K + 8 * ch;
+    soff[q] = row * LDS_ROW + 8 * ch;
+  }
+  u4 st[2][4];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      st[q][0] = *reinterpret_cast<const u4*>(Ah + goff[q][0] + k0);
+      st[q][1] = *reinterpret_cast<const u4*>(Al + goff[q][0] + k0);
+      st[q][2] = *reinterpret_cast<const u4*>(Bh + goff[q][1] + k0);
+      st[q][3] = *reinterpret_cast<const u4*>(Bl + goff[q][1] + k0);
+    }
+  };
+  auto sstore = [&](int stage) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int a = 0; a < 4; ++a) *reinterpret_cast<u4*>(&sm[stage][a][soff[q]]) = st[q][a];
+  };
+
+  f4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  int cur = 0;
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    const bool more = k0 + 32 < K;
+    if (more) gload(k0 + 32);
+    u4 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int ra = (wr + 16 * a + i) * LDS_ROW + 8 * g;
+      const int rb = (wc + 16 * a + i) * LDS_ROW + 8 * g;
+      ah[a] = *reinterpret_cast<const u4*>(&sm[cur][0][ra]);
+      al[a] = *reinterpret_cast<const u4*>(&sm[cur][1][ra]);
+      bh[a] = *reinterpret_cast<const u4*>(&sm[cur][2][rb]);
+      bl[a] = *reinterpret_cast<const u4*>(&sm[cur][3][rb]);
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(al[a]), as_bf16x8(bh[b]), acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(ah[a]), as_bf16x8(bl[b]), acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(ah[a]), as_bf16x8(bh[b]), acc[a][b], 0, 0, 0);
+      }
+    if (more) sstore(cur ^ 1);
+    __syncthreads();
+    cur ^= 1;
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int col = tn * 128 + wc + 16 * b + i;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = tm * 128 + wr + 16 * a + 4 * g + v;
+        if (row < M && col < N) C[(size_t)row * ldc + col] = acc[a][b][v];
+      }
+    }
+}
+
+// OAGPU_GEMM_VARIANT=direct selects the register-fed kernels (for
+// comparisons); the default is the LDS-staged one where it exists.
+static int gemm_variant() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = getenv("OAGPU_GEMM_VARIANT");
+    v = (e && !strcmp(e, "direct")) ? 1 : 0;
+  }
+  return v;
+}
+
 void launch_gemm(int prec, const void* Ahi, const void* Alo, const void* Bhi, const void* Blo,
                  void* C, int M, int N, int K, int ldc, hipStream_t st) {
   const int tiles_m = (M + 127) / 128, tiles_n = (N + 127) / 128;
@@ -205,6 +315,10 @@ void launch_gemm(int prec, const void* Ahi, const void* Alo, const void* Bhi, co
   else if (prec == PREC_FP32)
     hipLaunchKernelGGL(k_gemm_f32, grid, block, 0, st, (const float*)Ahi, (const float*)Bhi,
                        (float*)C, M, N, K, ldc, tiles_n, total);
+  else if (gemm_variant() == 0)
+    hipLaunchKernelGGL(k_gemm_bf16x3_lds, grid, block, 0, st, (const uint16_t*)Ahi, (const uint16_t*)Alo,
+                       (const uint16_t*)Bhi, (const uint16_t*)Blo, (float*)C, M, N, K, ldc, tiles_n,
+                       total);
   else
     hipLaunchKernelGGL(k_gemm_bf16x3, grid, block, 0, st, (const uint16_t*)Ahi, (const uint16_t*)Alo,
                        (const uint16_t*)Bhi, (const uint16_t*)Blo, (float*)C, M, N, K, ldc, tiles_n,

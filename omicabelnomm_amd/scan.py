@@ -68,7 +68,54 @@ class Outputs:
     timings_ms: dict = field(default_factory=dict)
 
 
+class _CSettings(ctypes.Structure):
+    _fields_ = [
+        ("fnameAL", ctypes.c_char_p), ("fnameAR", ctypes.c_char_p), ("fnameY", ctypes.c_char_p),
+        ("fnameOutFiles", ctypes.c_char_p), ("fname_excludelist", ctypes.c_char_p),
+        ("fname_interactions", ctypes.c_char_p), ("r", ctypes.c_int), ("threads", ctypes.c_int),
+        ("storeBin", ctypes.c_int), ("storePInd", ctypes.c_int), ("disp_cov", ctypes.c_int),
+        ("minPdisp", ctypes.c_float), ("minPstore", ctypes.c_float), ("minR2disp", ctypes.c_float),
+        ("minR2store", ctypes.c_float), ("use_interactions", ctypes.c_int),
+        ("keep_depVar", ctypes.c_int), ("use_multiple_interaction_sets", ctypes.c_int),
+        ("ini_IDX_interactions", ctypes.c_int), ("end_IDX_interactions", ctypes.c_int),
+        ("limit_m", ctypes.c_int), ("limit_t", ctypes.c_int), ("limit_n", ctypes.c_int),
+        ("precision", ctypes.c_int), ("device", ctypes.c_int), ("verbose", ctypes.c_int),
+    ]
+
+
+class _COutputs(ctypes.Structure):
+    _fields_ = [
+        ("duration", ctypes.c_double), ("acc_loady", ctypes.c_double),
+        ("acc_loadxr", ctypes.c_double), ("acc_gpu", ctypes.c_double),
+        ("acc_storeb", ctypes.c_double), ("total_sig_results", ctypes.c_longlong),
+        ("n_txt_lines", ctypes.c_longlong), ("n_bin_results", ctypes.c_longlong),
+        ("n", ctypes.c_int), ("l", ctypes.c_int), ("r", ctypes.c_int), ("t", ctypes.c_int),
+        ("m", ctypes.c_int), ("realN", ctypes.c_int),
+    ]
+
+
+_host = None
+
+
+def _host_lib():
+    global _host
+    if _host is None:
+        import os
+        path = os.path.join(L.HERE, "libomicabelnomm.so")
+        if not os.path.exists(path):
+            raise RuntimeError("libomicabelnomm.so is not built; run omicabelnomm_amd.build.build()")
+        L.load()
+        h = ctypes.CDLL(path)
+        h.omicabelnomm_solve.argtypes = [ctypes.POINTER(_CSettings), ctypes.POINTER(_COutputs),
+                                         ctypes.c_char_p, ctypes.c_int]
+        h.omicabelnomm_solve.restype = ctypes.c_int
+        _host = h
+    return _host
+
+
 class Algorithm:
+    """The reference's entry points (src/Algorithm.h:41-47)."""
+
     def applyDefaultParams(self, params: Settings) -> None:
         """src/Algorithm.cpp:277-311."""
         d = Settings()
@@ -77,6 +124,39 @@ class Algorithm:
                   "fname_interactions", "ini_IDX_interactions", "end_IDX_interactions",
                   "minR2store", "minR2disp", "minPstore", "minPdisp", "limit_m", "limit_t", "limit_n"):
             setattr(params, k, getattr(d, k))
+
+    def solve(self, params: Settings, out: Outputs, type: int = P_NEQ_B_OPT_MD, verbose=False) -> None:
+        """Drop-in file scan: DatABEL inputs named in params, outputs
+        <fnameOutFiles>_dis.txt (+ _sto.ibin/_sto.dbin with storeBin).
+        Raises OagpuError where the reference prints and exits."""
+        if type != P_NEQ_B_OPT_MD:
+            return   # the reference's solve ignores every other id (src/Algorithm.cpp:37-48)
+        h = _host_lib()
+        enc = lambda x: (x or "").encode()
+        cs = _CSettings(
+            fnameAL=enc(params.fnameAL), fnameAR=enc(params.fnameAR), fnameY=enc(params.fnameY),
+            fnameOutFiles=enc(params.fnameOutFiles), fname_excludelist=enc(params.fname_excludelist),
+            fname_interactions=enc(params.fname_interactions), r=params.r, threads=params.threads,
+            storeBin=int(params.storeBin), storePInd=int(params.storePInd),
+            disp_cov=int(params.disp_cov), minPdisp=params.minPdisp, minPstore=params.minPstore,
+            minR2disp=params.minR2disp, minR2store=params.minR2store,
+            use_interactions=int(params.use_interactions), keep_depVar=int(params.keep_depVar),
+            use_multiple_interaction_sets=int(params.use_multiple_interaction_sets),
+            ini_IDX_interactions=params.ini_IDX_interactions,
+            end_IDX_interactions=params.end_IDX_interactions, limit_m=params.limit_m,
+            limit_t=params.limit_t, limit_n=params.limit_n,
+            precision=L.PRECISIONS[params.precision], device=params.device, verbose=int(verbose))
+        co = _COutputs()
+        err = ctypes.create_string_buffer(4096)
+        rc = h.omicabelnomm_solve(ctypes.byref(cs), ctypes.byref(co), err, 4096)
+        if rc != 0:
+            raise L.OagpuError(rc, err.value.decode(errors="replace"))
+        out.duration = co.duration
+        out.total_sig_results = co.total_sig_results
+        out.n_hits = co.n_bin_results
+        for k in ("n", "l", "r", "t", "m"):
+            setattr(params, k, getattr(co, k))
+        params.p = co.l + co.r
 
 
 def normalize_thresholds(params: Settings) -> Settings:
