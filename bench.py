@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--p-store", type=float, default=1e-6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run one 1 GiB device copy first (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     return ap.parse_args()
 
 
@@ -121,6 +123,12 @@ def main():
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.copy_probe:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     over = {}
     if args.m:
         over["m"] = args.m
@@ -140,6 +148,7 @@ def main():
     gemm_tflops = (flops / chunks) / (gemm_avg_ms * 1e-3) / 1e12
     hits_mixed = res.hits
     timings = res.timings_ms
+    variant = sc.gemm_variant()
     sc.close()
 
     fp64 = None
@@ -147,9 +156,10 @@ def main():
     if args.fp64_steps > 0 and args.precision != "fp64":
         sc = make_scanner(data, "fp64", args.p_store)
         dt64, res64, gemm64_ms, chunks64 = timed(sc, data, args.fp64_steps, 0)
+        variant64 = sc.gemm_variant()
         sc.close()
         fp64 = {"value": tests / dt64, "ms_per_step": dt64 * 1e3,
-                "gemm_tflops": (flops / chunks64) / (gemm64_ms * 1e-3) / 1e12,
+                "gemm_tflops": (flops / chunks64) / (gemm64_ms * 1e-3) / 1e12, "gemm_variant": variant64,
                 "timings_ms": res64.timings_ms}
         # hit set at p < 1e-8 and the -log10 p / beta agreement on shared hits
         key = lambda h: set(zip(h["trait"][h["p"] < 1e-8], h["snp"][h["p"] < 1e-8], h["h"][h["p"] < 1e-8]))
@@ -184,7 +194,7 @@ def main():
             "gen_seconds": round(gen_s, 2)},
         "roofline": {"bound": "mfma", "achieved": gemm_tflops, "peak": peak, "unit": "TFLOP/s",
                      "frac": gemm_tflops / peak, "traffic": None,
-                     "kernel": "k_gemm_%s" % args.precision,
+                     "kernel": variant,
                      "note": "algorithmic 2*n*r*m*(t+l) flops / HIP-event time of the contraction"},
         "cpu_baseline": cpu,
         "fp64": fp64,

@@ -153,6 +153,10 @@ int oagpu_scan(oagpu_ctx *ctx, oagpu_dense *dense, oagpu_hit *hits,
  * [5] fallback, [6] whole scan.  Returns the number of values written. */
 int oagpu_last_timings(const oagpu_ctx *ctx, double *out, int n);
 
+/* Name of the contraction kernel variant the autotuner chose for the last
+ * oagpu_scan (no counterpart in the reference, which calls BLAS). */
+const char *oagpu_gemm_variant(const oagpu_ctx *ctx);
+
 /* getTvalue (src/Utility.cpp:244-250): sqrt(2) erfinv(1 - p) as a float. */
 double oagpu_tvalue(float pval);
 

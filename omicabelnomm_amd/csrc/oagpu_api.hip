@@ -53,6 +53,8 @@ struct oagpu_ctx {
   unsigned long long* hash = nullptr;
   int* scratch_i = nullptr;   // rep_of / mismatch
   int G = 0, max_pat_cnt = 0;
+  unsigned long long* rowmask = nullptr;   // per row: patterns it is missing in (G <= 64)
+  bool use_rowmask = false;
   int *pat_cnt = nullptr, *pat_rep = nullptr, *pat_pd = nullptr;
   double *UA = nullptr, *UA_invd = nullptr;
   size_t pat_cap = 0;
@@ -81,6 +83,7 @@ struct oagpu_ctx {
   FallbackEntry* fb = nullptr;
   int fb_cap = 0;
   double timings[8] = {0};
+  std::string gemm_variant = "none";
 };
 
 #define HIPCK(ctx, expr)                                                              \
@@ -279,6 +282,7 @@ extern "C" int oagpu_create(const oagpu_params* prm, oagpu_ctx** out) {
   HIPCK(c, hipMalloc(&c->rt, sizeof(double) * (size_t)l * c->ldt));
   HIPCK(c, hipMalloc(&c->zt, sizeof(double) * (size_t)l * c->ldt));
   HIPCK(c, hipMalloc(&c->hash, sizeof(unsigned long long) * 2 * T));
+  HIPCK(c, hipMalloc(&c->rowmask, sizeof(unsigned long long) * (size_t)n));
   return OAGPU_OK;
 }
 
@@ -288,7 +292,7 @@ extern "C" void oagpu_destroy(oagpu_ctx* c) {
   void* bufs[] = {c->XL_cm, c->XL_rm, c->U0, c->Stl_inv_d, c->Stl_pk, c->XLop_hi, c->XLop_lo, c->E,
                   c->Y_own, c->Bhi, c->Blo, c->rows, c->ymiss, c->yzero, c->ypat, c->yy, c->sst,
                   c->rss0, c->rt, c->zt, c->hash, c->scratch_i, c->pat_cnt, c->pat_rep, c->pat_pd,
-                  c->UA, c->UA_invd, c->G_own, c->col_src, c->col_env, c->Ahi, c->Alo, c->Cbuf,
+                  c->UA, c->UA_invd, c->rowmask, c->G_own, c->col_src, c->col_env, c->Ahi, c->Alo, c->Cbuf,
                   c->col_mean, c->Sbr, c->Str, c->varx, c->Ccorr, c->hits, c->counters, c->fb};
   for (void* b : bufs) dfree(b);
   if (c->st) (void)hipStreamDestroy(c->st);
@@ -496,6 +500,13 @@ extern "C" int oagpu_load_traits(oagpu_ctx* c, const void* Y, int dtype, int whe
   d = make_dev(c);
   launch_pattern(d, c->Stl_pk, G, c->pat_rep, c->UA, c->UA_invd, c->pat_pd, c->st);
   HIPCK(c, hipGetLastError());
+  // Few patterns: corrections are accumulated inside the operand pass.
+  c->use_rowmask = c->max_pat_cnt > 0 && G <= 64 && prep_corr_lds_bytes(G, c->Q) <= 65536;
+  if (c->use_rowmask) {
+    HIPCK(c, hipMemsetAsync(c->rowmask, 0, sizeof(unsigned long long) * (size_t)n, c->st));
+    launch_rowmask(G, n, c->pat_cnt, c->pat_rep, c->rows, c->rowmask, c->st);
+    HIPCK(c, hipGetLastError());
+  }
   d = make_dev(c);
   launch_trait_solve(d, c->lb, c->zt, c->rss0, c->st);
   HIPCK(c, hipGetLastError());
@@ -564,7 +575,8 @@ extern "C" int oagpu_load_genotypes(oagpu_ctx* c, const void* G, int dtype, int 
 
 static int ensure_chunk(oagpu_ctx* c, int mc) {
   const int r = c->r, l = c->l, p = c->p, G = c->G, Q = c->Q;
-  const size_t arows = (size_t)((mc * r + 127) / 128 * 128);
+  const size_t pad = (size_t)gemm_row_pad();
+  const size_t arows = ((size_t)mc * r + pad - 1) / pad * pad;
   const size_t abytes = arows * c->n_pad * c->op_size;
   const size_t cbytes = (size_t)mc * r * (c->t + l) * c->c_size;
   const size_t qbytes = sizeof(double) * (size_t)mc * Q * std::max(G, 1);
@@ -633,14 +645,14 @@ extern "C" int oagpu_scan(oagpu_ctx* c, oagpu_dense* dense, oagpu_hit* hits, lon
     d.mc = mcc;
     hipEvent_t* E = &ev[(size_t)ci * 7];
     HIPCK(c, hipEventRecord(E[0], c->st));
-    launch_prep(d, c->prm.precision, c->Ahi, c->Alo, c->st);
+    launch_prep(d, c->prm.precision, c->Ahi, c->Alo, c->use_rowmask ? c->rowmask : nullptr, c->st);
     HIPCK(c, hipEventRecord(E[1], c->st));
     launch_gemm(c->prm.precision, c->Ahi, c->Alo, c->Bhi, c->Blo, c->Cbuf, mcc * r, t + l, c->n_pad, t + l,
                 c->st);
     HIPCK(c, hipEventRecord(E[2], c->st));
     launch_snp_gram(d, c->prm.precision, c->lb, c->st);
     HIPCK(c, hipEventRecord(E[3], c->st));
-    if (c->max_pat_cnt > 0) launch_mask(d, c->lb, c->st);
+    if (c->max_pat_cnt > 0 && !c->use_rowmask) launch_mask(d, c->lb, c->st);
     HIPCK(c, hipEventRecord(E[4], c->st));
     launch_reset_counter(c->counters, CNT_FB_CHUNK, c->st);
     launch_epilogue(d, c->prm.precision, c->lb, c->st);
@@ -649,6 +661,7 @@ extern "C" int oagpu_scan(oagpu_ctx* c, oagpu_dense* dense, oagpu_hit* hits, lon
     HIPCK(c, hipEventRecord(E[6], c->st));
     HIPCK(c, hipGetLastError());
   }
+  c->gemm_variant = gemm_variant_name(c->prm.precision, (int)std::min<long long>(mc, m) * r, t + l, c->n_pad);
   unsigned long long cnt[CNT_N];
   HIPCK(c, hipMemcpyAsync(cnt, c->counters, sizeof(cnt), hipMemcpyDeviceToHost, c->st));
   HIPCK(c, hipStreamSynchronize(c->st));
@@ -723,3 +736,5 @@ extern "C" int oagpu_last_timings(const oagpu_ctx* c, double* out, int n) {
   for (int i = 0; i < k; ++i) out[i] = c->timings[i];
   return k;
 }
+
+extern "C" const char* oagpu_gemm_variant(const oagpu_ctx* c) { return c ? c->gemm_variant.c_str() : ""; }

@@ -22,6 +22,9 @@
 // row block run on the same XCD and reuse it from that XCD's L2.
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "oagpu_kernels.h"
 
@@ -293,36 +296,248 @@ __global__ __launch_bounds__(256, 1) void k_gemm_bf16x3_lds(const uint16_t* __re
     }
 }
 
-// OAGPU_GEMM_VARIANT=direct selects the register-fed kernels (for
-// comparisons); the default is the LDS-staged one where it exists.
-static int gemm_variant() {
-  static int v = -1;
-  if (v < 0) {
-    const char* e = getenv("OAGPU_GEMM_VARIANT");
-    v = (e && !strcmp(e, "direct")) ? 1 : 0;
+
+// Larger tiles for the split-BF16 contraction: workgroup 256 x 128, wave
+// 128 x 64 (8 x 4 MFMA blocks), so each LDS byte a wave reads feeds twice
+// as many MFMAs as in the 128 x 128 kernel.  The three products are issued
+// product-major, so consecutive MFMAs never share an accumulator.
+template <int WM_BLK>
+__global__ __launch_bounds__(256, 1) void k_gemm_bf16x3_big(const uint16_t* __restrict__ Ah,
+                                                           const uint16_t* __restrict__ Al,
+                                                           const uint16_t* __restrict__ Bh,
+                                                           const uint16_t* __restrict__ Bl,
+                                                           float* __restrict__ C, int M, int N, int K,
+                                                           int ldc, int tiles_n, int total) {
+  constexpr int BM = 32 * WM_BLK;          // rows of A per workgroup (2 waves x 16*WM_BLK)
+  constexpr int BN = 128;
+  constexpr int A_CH = BM * 4 / 256;       // 16-byte chunks per thread per array
+  constexpr int B_CH = BN * 4 / 256;
+  __shared__ __attribute__((aligned(16))) uint16_t sa[2][2][BM * LDS_ROW];
+  __shared__ __attribute__((aligned(16))) uint16_t sb[2][2][BN * LDS_ROW];
+  const int tile = xcd_tile(blockIdx.x, total);
+  const int tm = tile / tiles_n, tn = tile - tm * tiles_n;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
+  const int i = lane & 15, g = lane >> 4;
+  const int wr = (w & 1) * (16 * WM_BLK), wc = (w >> 1) * 64;
+
+  size_t ga[A_CH], gb[B_CH];
+  int sa_off[A_CH], sb_off[B_CH];
+#pragma unroll
+  for (int q = 0; q < A_CH; ++q) {
+    const int c = tid + q * 256, row = c >> 2, ch = c & 3;
+    ga[q] = (size_t)(tm * BM + row) * K + 8 * ch;
+    sa_off[q] = row * LDS_ROW + 8 * ch;
   }
-  return v;
+#pragma unroll
+  for (int q = 0; q < B_CH; ++q) {
+    const int c = tid + q * 256, row = c >> 2, ch = c & 3;
+    gb[q] = (size_t)(tn * BN + row) * K + 8 * ch;
+    sb_off[q] = row * LDS_ROW + 8 * ch;
+  }
+  u4 sta[A_CH][2], stb[B_CH][2];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < A_CH; ++q) {
+      sta[q][0] = *reinterpret_cast<const u4*>(Ah + ga[q] + k0);
+      sta[q][1] = *reinterpret_cast<const u4*>(Al + ga[q] + k0);
+    }
+#pragma unroll
+    for (int q = 0; q < B_CH; ++q) {
+      stb[q][0] = *reinterpret_cast<const u4*>(Bh + gb[q] + k0);
+      stb[q][1] = *reinterpret_cast<const u4*>(Bl + gb[q] + k0);
+    }
+  };
+  auto sstore = [&](int stage) {
+#pragma unroll
+    for (int q = 0; q < A_CH; ++q) {
+      *reinterpret_cast<u4*>(&sa[stage][0][sa_off[q]]) = sta[q][0];
+      *reinterpret_cast<u4*>(&sa[stage][1][sa_off[q]]) = sta[q][1];
+    }
+#pragma unroll
+    for (int q = 0; q < B_CH; ++q) {
+      *reinterpret_cast<u4*>(&sb[stage][0][sb_off[q]]) = stb[q][0];
+      *reinterpret_cast<u4*>(&sb[stage][1][sb_off[q]]) = stb[q][1];
+    }
+  };
+
+  f4 acc[WM_BLK][4];
+#pragma unroll
+  for (int a = 0; a < WM_BLK; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  int cur = 0;
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    const bool more = k0 + 32 < K;
+    if (more) gload(k0 + 32);
+    u4 bh[4], bl[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int rb = (wc + 16 * b + i) * LDS_ROW + 8 * g;
+      bh[b] = *reinterpret_cast<const u4*>(&sb[cur][0][rb]);
+      bl[b] = *reinterpret_cast<const u4*>(&sb[cur][1][rb]);
+    }
+#pragma unroll
+    for (int a = 0; a < WM_BLK; ++a) {
+      const int ra = (wr + 16 * a + i) * LDS_ROW + 8 * g;
+      const u4 ah = *reinterpret_cast<const u4*>(&sa[cur][0][ra]);
+      const u4 al = *reinterpret_cast<const u4*>(&sa[cur][1][ra]);
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(al), as_bf16x8(bh[b]), acc[a][b], 0, 0, 0);
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(ah), as_bf16x8(bl[b]), acc[a][b], 0, 0, 0);
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(ah), as_bf16x8(bh[b]), acc[a][b], 0, 0, 0);
+    }
+    if (more) sstore(cur ^ 1);
+    __syncthreads();
+    cur ^= 1;
+  }
+#pragma unroll
+  for (int a = 0; a < WM_BLK; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int col = tn * BN + wc + 16 * b + i;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = tm * BM + wr + 16 * a + 4 * g + v;
+        if (row < M && col < N) C[(size_t)row * ldc + col] = acc[a][b][v];
+      }
+    }
 }
+
+// ---------------------------------------------------------------------------
+// Kernel variants and the autotuner.  Each precision has a list of
+// variants; the first launch for a (precision, M, N, K) shape class times
+// every variant on the real operands (HIP events on the caller's stream)
+// and caches the fastest.  OAGPU_GEMM_VARIANT=<name> pins one.
+struct GemmVariant {
+  const char* name;
+  int prec;
+  int bm, bn;   // workgroup tile
+  void (*launch)(const void*, const void*, const void*, const void*, void*, int, int, int, int, hipStream_t);
+};
+
+static void v_f64(const void* Ah, const void*, const void* Bh, const void*, void* C, int M, int N, int K,
+                  int ldc, hipStream_t st) {
+  const int tiles_m = (M + 127) / 128, tiles_n = (N + 127) / 128, total = tiles_m * tiles_n;
+  hipLaunchKernelGGL(k_gemm_f64, dim3(total), dim3(256), 0, st, (const double*)Ah, (const double*)Bh,
+                     (double*)C, M, N, K, ldc, tiles_n, total);
+}
+static void v_f32(const void* Ah, const void*, const void* Bh, const void*, void* C, int M, int N, int K,
+                  int ldc, hipStream_t st) {
+  const int tiles_m = (M + 127) / 128, tiles_n = (N + 127) / 128, total = tiles_m * tiles_n;
+  hipLaunchKernelGGL(k_gemm_f32, dim3(total), dim3(256), 0, st, (const float*)Ah, (const float*)Bh,
+                     (float*)C, M, N, K, ldc, tiles_n, total);
+}
+static void v_bf_direct(const void* Ah, const void* Al, const void* Bh, const void* Bl, void* C, int M, int N,
+                        int K, int ldc, hipStream_t st) {
+  const int tiles_m = (M + 127) / 128, tiles_n = (N + 127) / 128, total = tiles_m * tiles_n;
+  hipLaunchKernelGGL(k_gemm_bf16x3, dim3(total), dim3(256), 0, st, (const uint16_t*)Ah, (const uint16_t*)Al,
+                     (const uint16_t*)Bh, (const uint16_t*)Bl, (float*)C, M, N, K, ldc, tiles_n, total);
+}
+static void v_bf_lds(const void* Ah, const void* Al, const void* Bh, const void* Bl, void* C, int M, int N,
+                     int K, int ldc, hipStream_t st) {
+  const int tiles_m = (M + 127) / 128, tiles_n = (N + 127) / 128, total = tiles_m * tiles_n;
+  hipLaunchKernelGGL(k_gemm_bf16x3_lds, dim3(total), dim3(256), 0, st, (const uint16_t*)Ah,
+                     (const uint16_t*)Al, (const uint16_t*)Bh, (const uint16_t*)Bl, (float*)C, M, N, K, ldc,
+                     tiles_n, total);
+}
+static void v_bf_big(const void* Ah, const void* Al, const void* Bh, const void* Bl, void* C, int M, int N,
+                     int K, int ldc, hipStream_t st) {
+  const int tiles_m = (M + 255) / 256, tiles_n = (N + 127) / 128, total = tiles_m * tiles_n;
+  hipLaunchKernelGGL(k_gemm_bf16x3_big<8>, dim3(total), dim3(256), 0, st, (const uint16_t*)Ah,
+                     (const uint16_t*)Al, (const uint16_t*)Bh, (const uint16_t*)Bl, (float*)C, M, N, K, ldc,
+                     tiles_n, total);
+}
+
+static const GemmVariant kVariants[] = {
+    {"f64_direct_128x128", PREC_FP64, 128, 128, v_f64},
+    {"f32_direct_128x128", PREC_FP32, 128, 128, v_f32},
+    {"bf16x3_lds_256x128", PREC_BF16X3, 256, 128, v_bf_big},
+    {"bf16x3_lds_128x128", PREC_BF16X3, 128, 128, v_bf_lds},
+    {"bf16x3_direct_128x128", PREC_BF16X3, 128, 128, v_bf_direct},
+};
+constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+
+struct TuneKey {
+  int prec, mclass, n, k;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(prec, mclass, n, k) < std::tie(o.prec, o.mclass, o.n, o.k);
+  }
+};
+static std::map<TuneKey, int> g_tuned;
+static std::mutex g_tune_mu;
+
+static int pinned_variant(int prec) {
+  const char* e = getenv("OAGPU_GEMM_VARIANT");
+  if (!e) return -1;
+  for (int v = 0; v < kNumVariants; ++v)
+    if (kVariants[v].prec == prec && !strcmp(kVariants[v].name, e)) return v;
+  return -1;
+}
+
+const char* gemm_variant_name(int prec, int M, int N, int K) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  const int mclass = M >= 65536 ? 2 : (M >= 4096 ? 1 : 0);
+  auto it = g_tuned.find(TuneKey{prec, mclass, N, K});
+  return it == g_tuned.end() ? "untuned" : kVariants[it->second].name;
+}
+
+int gemm_row_pad() { return 256; }
 
 void launch_gemm(int prec, const void* Ahi, const void* Alo, const void* Bhi, const void* Blo,
                  void* C, int M, int N, int K, int ldc, hipStream_t st) {
-  const int tiles_m = (M + 127) / 128, tiles_n = (N + 127) / 128;
-  const int total = tiles_m * tiles_n;
-  dim3 grid(total), block(256);
-  if (prec == PREC_FP64)
-    hipLaunchKernelGGL(k_gemm_f64, grid, block, 0, st, (const double*)Ahi, (const double*)Bhi,
-                       (double*)C, M, N, K, ldc, tiles_n, total);
-  else if (prec == PREC_FP32)
-    hipLaunchKernelGGL(k_gemm_f32, grid, block, 0, st, (const float*)Ahi, (const float*)Bhi,
-                       (float*)C, M, N, K, ldc, tiles_n, total);
-  else if (gemm_variant() == 0)
-    hipLaunchKernelGGL(k_gemm_bf16x3_lds, grid, block, 0, st, (const uint16_t*)Ahi, (const uint16_t*)Alo,
-                       (const uint16_t*)Bhi, (const uint16_t*)Blo, (float*)C, M, N, K, ldc, tiles_n,
-                       total);
-  else
-    hipLaunchKernelGGL(k_gemm_bf16x3, grid, block, 0, st, (const uint16_t*)Ahi, (const uint16_t*)Alo,
-                       (const uint16_t*)Bhi, (const uint16_t*)Blo, (float*)C, M, N, K, ldc, tiles_n,
-                       total);
+  int v = pinned_variant(prec);
+  if (v < 0) {
+    const int mclass = M >= 65536 ? 2 : (M >= 4096 ? 1 : 0);
+    const TuneKey key{prec, mclass, N, K};
+    {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      auto it = g_tuned.find(key);
+      if (it != g_tuned.end()) v = it->second;
+    }
+    int nvar = 0, only = -1;
+    for (int c = 0; c < kNumVariants; ++c)
+      if (kVariants[c].prec == prec) {
+        ++nvar;
+        only = c;
+      }
+    if (v < 0 && nvar == 1) v = only;
+    if (v < 0) {
+      // time every variant of this precision on these operands
+      float best = 1e30f;
+      hipEvent_t e0, e1;
+      (void)hipEventCreate(&e0);
+      (void)hipEventCreate(&e1);
+      for (int c = 0; c < kNumVariants; ++c) {
+        if (kVariants[c].prec != prec) continue;
+        kVariants[c].launch(Ahi, Alo, Bhi, Blo, C, M, N, K, ldc, st);   // warm
+        (void)hipEventRecord(e0, st);
+        kVariants[c].launch(Ahi, Alo, Bhi, Blo, C, M, N, K, ldc, st);
+        (void)hipEventRecord(e1, st);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best) {
+          best = ms;
+          v = c;
+        }
+      }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      g_tuned[key] = v;
+    }
+  }
+  kVariants[v].launch(Ahi, Alo, Bhi, Blo, C, M, N, K, ldc, st);
 }
 
 }  // namespace oa

@@ -15,7 +15,13 @@ void launch_pattern(const DevScan& d, const double* Stl_pk, int G, const int* pa
 void launch_trait_solve(const DevScan& d, int lb, double* zt, double* rss0, hipStream_t st);
 
 // SNP side (oagpu_kernels_snp.hip)
-void launch_prep(const DevScan& d, int prec, void* Ahi, void* Alo, hipStream_t st);
+// rowmask (nullptr = no fused corrections): bit g of rowmask[k] is set when
+// row k is missing in pattern g (G <= 64).
+void launch_prep(const DevScan& d, int prec, void* Ahi, void* Alo, const unsigned long long* rowmask,
+                 hipStream_t st);
+size_t prep_corr_lds_bytes(int G, int Q);   // LDS the fused corrections need
+void launch_rowmask(int G, int n, const int* pat_cnt, const int* pat_rep, const int* rows,
+                    unsigned long long* mask, hipStream_t st);
 void launch_snp_gram(const DevScan& d, int prec, int lb, hipStream_t st);
 void launch_mask(const DevScan& d, int lb, hipStream_t st);
 void launch_epilogue(const DevScan& d, int prec, int lb, hipStream_t st);
@@ -23,9 +29,11 @@ void launch_fallback(const DevScan& d, int lb, hipStream_t st);
 void launch_reset_counter(unsigned long long* c, int idx, hipStream_t st);
 
 // contraction (oagpu_gemm.hip): C[c][j] = sum_k A[c][k] B[j][k]
-// A: M rows of K (padded to 128 rows), B: N rows of K (padded to 128 rows),
-// K a multiple of 64.
+// A: M rows of K (padded to gemm_row_pad() rows), B: N rows of K (padded
+// to 128 rows), K a multiple of 64.  The variant is chosen by timing.
 void launch_gemm(int prec, const void* Ahi, const void* Alo, const void* Bhi, const void* Blo,
                  void* C, int M, int N, int K, int ldc, hipStream_t st);
+const char* gemm_variant_name(int prec, int M, int N, int K);
+int gemm_row_pad();
 
 }  // namespace oa

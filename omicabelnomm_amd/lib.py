@@ -68,7 +68,7 @@ class Stats(ctypes.Structure):
 
 EXPORTS = ["oagpu_create", "oagpu_destroy", "oagpu_last_error", "oagpu_set_covariates",
            "oagpu_set_interactions", "oagpu_load_traits", "oagpu_load_genotypes", "oagpu_scan",
-           "oagpu_last_timings", "oagpu_tvalue"]
+           "oagpu_last_timings", "oagpu_gemm_variant", "oagpu_tvalue"]
 
 _lib = None
 
@@ -94,6 +94,8 @@ def load():
     L.oagpu_load_genotypes.argtypes = [vp, vp, i, i, ll, i, i, vp, vp, i]
     L.oagpu_scan.argtypes = [vp, ctypes.POINTER(Dense), vp, ll, ctypes.POINTER(Stats)]
     L.oagpu_last_timings.argtypes = [vp, ctypes.POINTER(ctypes.c_double), i]
+    L.oagpu_gemm_variant.argtypes = [vp]
+    L.oagpu_gemm_variant.restype = ctypes.c_char_p
     L.oagpu_tvalue.argtypes = [ctypes.c_float]
     L.oagpu_tvalue.restype = ctypes.c_double
     for f in ["oagpu_set_covariates", "oagpu_set_interactions", "oagpu_load_traits",

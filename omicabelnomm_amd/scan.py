@@ -315,6 +315,9 @@ class Scanner:
         return ScanResult(hits=hits[: stats["n_hits"]].copy(), stats=stats, dense=dres,
                           timings_ms=self.timings())
 
+    def gemm_variant(self) -> str:
+        return self._L.oagpu_gemm_variant(self._ctx).decode()
+
     def timings(self) -> dict:
         buf = (ctypes.c_double * 7)()
         k = self._L.oagpu_last_timings(self._ctx, buf, 7)

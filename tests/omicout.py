@@ -95,3 +95,27 @@ def expected_dis_rows(o, names, l, r, min_p_disp, min_r2_disp, real_n, tb=1000, 
                                          T=o["tstat"][y, s, h], P=P, R2=float(R2),
                                          npct=o["n_used"][y] / real_n))
     return rows
+
+
+def expected_records(o, l, r, tb=1000, mb=500):
+    """The _sto.dbin records of a dense oracle result, in file order, each
+    as (block's first trait, AR offset, n_used, [(AL idx, AR idx, B, SE)])."""
+    t, m, p = o["beta"].shape
+    out = []
+    for y0 in range(0, t, tb):
+        for s0 in range(0, m, mb):
+            first = None
+            for y in range(y0, min(t, y0 + tb)):
+                for s in range(s0, min(m, s0 + mb)):
+                    if not o["emitted"][y, s]:
+                        continue
+                    if first is None:
+                        first = y
+                    res = []
+                    for h in range(p - 1, -1, -1):
+                        if o["kept"][y, s, h]:
+                            ii = s - s0
+                            res.append((255 if h >= l else h, ii * r + (h - l if h >= l else 0),
+                                        o["beta"][y, s, h], o["se"][y, s, h]))
+                    out.append((first, s0 * r, int(o["n_used"][y]), res))
+    return out

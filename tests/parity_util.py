@@ -44,6 +44,13 @@ def mlog10(p):
         return -np.log10(p)
 
 
+def dlog10p(pa, pb):
+    """|-log10 pa + log10 pb|, 0 where both underflowed to 0."""
+    d = np.abs(mlog10(pa) - mlog10(pb))
+    d[(pa == 0) & (pb == 0)] = 0.0
+    return d
+
+
 def compare_fp64(g, o, *, rtol=1e-10):
     """FP64 parity: beta and SE within rtol (relative to max(|beta|, SE)
     for beta, to SE for SE), masks and the kept/emitted sets identical."""
@@ -60,7 +67,7 @@ def compare_fp64(g, o, *, rtol=1e-10):
     assert err_b.max(initial=0) <= rtol, ("beta", err_b.max())
     assert err_se.max(initial=0) <= rtol, ("se", err_se.max())
     kept = o["kept"].astype(bool)
-    dl = np.abs(mlog10(d["pval"][kept]) - mlog10(o["pval"][kept]))
+    dl = dlog10p(d["pval"][kept], o["pval"][kept])
     assert dl.max(initial=0) <= 1e-8, ("-log10 p", dl.max())
     r2ok = ~np.isnan(o["r2"])
     np.testing.assert_allclose(d["r2"][r2ok], o["r2"][r2ok], rtol=1e-9, atol=1e-12)
@@ -88,5 +95,5 @@ def precision_report(g, o):
     from oracle.oracle import pvalue
     pv_o = np.vectorize(pvalue)(o["tstat"][ok])
     pv_g = np.vectorize(pvalue)(d["tstat"][ok])
-    dl = np.abs(mlog10(pv_g) - mlog10(pv_o))
+    dl = dlog10p(pv_g, pv_o)
     return float(db.max(initial=0)), float(dl.max(initial=0))

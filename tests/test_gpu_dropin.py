@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from omicabelnomm_amd import databel
-from omicout import expected_dis_rows, read_dbin, read_dis, read_ibin
+from omicout import expected_dis_rows, expected_records, read_dbin, read_dis, read_ibin
 from parity_util import compare_fp64
 from refmodel import CHECK_SETTINGS, GOLDEN, load_fixture, prepare, scenarios
 
@@ -103,14 +103,15 @@ def test_dropin_files_basic_model(oracle, fx, tmp_path):
     assert (ib["n"], ib["l"], ib["r"], ib["t"], ib["m"]) == (d["n"], 4, 1, 50, 100)
     assert ib["al_names"] == list(fx["xl_names"][1:]) and ib["y_names"] == list(fx["y_names"])
     recs = read_dbin(os.path.join(tmp, "basic_sto.dbin"), disp_cov=True, store_all=False, l=4, r=1)
-    assert len(recs) == o["n_emitted"]
+    want_recs = expected_records(o, l=4, r=1)
+    assert len(recs) == len(want_recs) == o["n_emitted"]
     assert sum(len(rc["results"]) for rc in recs) == o["n_kept"]
-    for rc in recs:
-        s_idx = rc["ar_offset"] + rc["results"][0][1]
-        assert rc["n_used"] == o["n_used"][rc["y"]]
-        for al, ar, B, SE in rc["results"]:
-            h = 4 if al == 255 else al
-            assert np.float32(o["beta"][rc["y"], s_idx, h]) == pytest.approx(B, rel=1e-6, abs=1e-7)
+    for rc, (blk_y, ar_off, n_used, res) in zip(recs, want_recs):
+        assert (rc["y"], rc["ar_offset"], rc["n_used"]) == (blk_y, ar_off, n_used)
+        assert [(a, b) for a, b, _, _ in rc["results"]] == [(a, b) for a, b, _, _ in res]
+        for (_, _, B, SE), (_, _, wB, wSE) in zip(rc["results"], res):
+            assert B == pytest.approx(np.float32(wB), rel=1e-6, abs=1e-7)
+            assert SE == pytest.approx(np.float32(wSE), rel=1e-6, abs=1e-7)
     # the CLI writes the same _dis.txt
     cli = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "omicabelnomm_amd", "omicabelnomm")
     cmd = [cli, "-c", os.path.join(tmp, "XL.fvi"), "-g", os.path.join(tmp, "XR.fvi"), "-p",
@@ -127,17 +128,17 @@ def test_dropin_files_basic_model(oracle, fx, tmp_path):
 def test_dropin_files_interactions(oracle, fx, tmp_path, multi):
     from omicabelnomm_amd import Algorithm, Outputs
     tmp = str(tmp_path)
-    _write_inputs(fx, tmp, ["INTR1"])
+    _write_inputs(fx, tmp, ["INTR"])
     s = _settings(tmp, "inter")
     s.storeBin = False
     s.use_interactions = True
     s.use_multiple_interaction_sets = multi
     s.keep_depVar = True
-    s.fname_interactions = os.path.join(tmp, "INTR1")
+    s.fname_interactions = os.path.join(tmp, "INTR")
     out = Outputs()
     Algorithm().solve(s, out)
     excl = open(os.path.join(GOLDEN, "exclude_individuals.txt")).read()
-    d = prepare(fx, "INTR1", multi=multi, keep_dep=True, excl_text=excl)
+    d = prepare(fx, "INTR", multi=multi, keep_dep=True, excl_text=excl)
     o = oracle.scan(d["XL"], d["XR"], d["Y"], d["r"], real_n=d["realN"], threads=4, **CHECK_SETTINGS)
     assert out.total_sig_results == o["n_sig"]
     assert (s.r, s.m) == (d["r"], d["m"])
