@@ -22,13 +22,14 @@
  * error, message in oagpu_last_error).  Matrices are column-major with a
  * leading dimension `ld` >= n (DatABEL .fvd order).  NaN marks a missing
  * value.  One context per host thread; every call is ordered on the
- * context's HIP stream and returns when its results are on the host.
- * Device pointers passed with OAGPU_DEVICE must stay valid until the next
- * oagpu_scan returns.
+ * context's HIP streams.  Device pointers passed with OAGPU_DEVICE, and
+ * host genotype blocks passed with OAGPU_HOST, must stay valid and
+ * unchanged until the scan that reads them has returned.
  */
 #ifndef OAGPU_H
 #define OAGPU_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -144,9 +145,26 @@ int oagpu_load_genotypes(oagpu_ctx *ctx, const void *G, int dtype, int where,
 
 /* Scan the loaded traits x SNPs.  Hits are written to hits[0..hit_cap)
  * sorted by (trait, snp, h descending), the reference's order inside one
- * block.  dense and hits may be NULL. */
+ * block.  dense and hits may be NULL.  Equivalent to oagpu_scan_async
+ * followed by oagpu_scan_wait. */
 int oagpu_scan(oagpu_ctx *ctx, oagpu_dense *dense, oagpu_hit *hits,
                long long hit_cap, oagpu_stats *stats);
+
+/* The same scan in two halves: oagpu_scan_async enqueues the kernels and
+ * returns; the next genotype block can then be uploaded (its copy overlaps
+ * this scan) before oagpu_scan_wait collects the results.  This pair
+ * replaces the reference's pthread hand-off between the compute loop and
+ * async_io (src/AIOwrapper.cpp:1674-1868). */
+int oagpu_scan_async(oagpu_ctx *ctx, int want_dense);
+int oagpu_scan_wait(oagpu_ctx *ctx, oagpu_dense *dense, oagpu_hit *hits,
+                    long long hit_cap, oagpu_stats *stats);
+
+/* Page-locked host buffers for genotype and trait blocks (hipHostMalloc):
+ * uploads from them run at full PCIe rate and asynchronously.  A host block
+ * passed to oagpu_load_genotypes may be refilled once the scan that reads
+ * it has returned from oagpu_scan / oagpu_scan_wait. */
+int oagpu_host_alloc(size_t bytes, void **ptr);
+void oagpu_host_free(void *ptr);
 
 /* Device-side timing of the last oagpu_scan, per kernel family (ms):
  * out[0] prep, [1] gemm, [2] snp gram, [3] corrections, [4] epilogue,

@@ -8,10 +8,10 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <functional>
-#include <future>
 #include <iostream>
 #include <memory>
 #include <sstream>
@@ -342,7 +342,7 @@ static int solve_impl(const omicabelnomm_settings* sin, omicabelnomm_outputs* ou
   gp.min_p_disp = s.minPdisp;
   gp.max_traits = P.tb;
   gp.max_snps = 0;
-  gp.max_hits = 1ll << 24;
+  gp.max_hits = 1ll << 22;
   gp.device = s.device;
   oagpu_ctx* ctx = nullptr;
   int rc = oagpu_create(&gp, &ctx);
@@ -374,25 +374,74 @@ static int solve_impl(const omicabelnomm_settings* sin, omicabelnomm_outputs* ou
   const size_t gelem = g64 ? 8 : 4;
   const int unit = P.multi ? P.file_block : mb_file;
   int gb_file = (int)std::max<size_t>(1, ((512ull << 20) / ((size_t)P.n * gelem * P.r_file)) / unit) * unit;
+  if (const char* e = std::getenv("OMICABELNOMM_BLOCK_SNPS"))   // smaller blocks (tests)
+    gb_file = std::max(unit, (std::atoi(e) / unit) * unit);
   gb_file = std::min(gb_file, ((P.m_file + unit - 1) / unit) * unit);
   const bool y64 = yfvi.type == 6;
 
   FvdReader yrd(fY + ".fvd", yfvi);
   FvdReader grd(fAR + ".fvd", arfvi);
-  std::vector<oagpu_hit> hits(1 << 20);
+  std::vector<oagpu_hit> hits((size_t)gp.max_hits);
   long long total_sig = 0;
   double acc_loady = 0, acc_loadxr = 0, acc_gpu = 0, acc_store = 0;
   const double t_start = now_s();
 
-  auto read_geno = [&](int f0, std::vector<unsigned char>& buf) {
+  // Two page-locked genotype blocks: while the GPU scans block f, the
+  // host reads block f+1 from disk into the other buffer and its upload is
+  // queued on the library's copy stream.
+  const size_t gblock_bytes = (size_t)gb_file * P.r_file * P.n * gelem;
+  void* hbuf[2] = {nullptr, nullptr};
+  for (int i = 0; i < 2; ++i)
+    if (oagpu_host_alloc(gblock_bytes, &hbuf[i]) != OAGPU_OK) throw Fail(1, "cannot allocate pinned host memory");
+  struct HostBufs {
+    void* p[2];
+    ~HostBufs() { oagpu_host_free(p[0]); oagpu_host_free(p[1]); }
+  } hold{{hbuf[0], hbuf[1]}};
+
+  auto read_geno = [&](int f0, void* buf) {
     const int nf = std::min(gb_file, P.m_file - f0);
-    buf.resize((size_t)nf * P.r_file * P.n * gelem);
     if (g64)
       grd.read_kept<double>((size_t)f0 * P.r_file, (size_t)nf * P.r_file, P.n_file, P.keep_rows, P.n,
-                            reinterpret_cast<double*>(buf.data()));
+                            reinterpret_cast<double*>(buf));
     else
       grd.read_kept<float>((size_t)f0 * P.r_file, (size_t)nf * P.r_file, P.n_file, P.keep_rows, P.n,
-                           reinterpret_cast<float*>(buf.data()));
+                           reinterpret_cast<float*>(buf));
+  };
+  auto upload = [&](int f0, void* buf) {
+    const int nf = std::min(gb_file, P.m_file - f0);
+    std::vector<int32_t> src, env;
+    int v0 = 0, nv = 0;
+    descriptors(P, 0, nf, src, env, v0, nv);
+    v0 = P.multi ? f0 * P.num_inter : f0;
+    const int r2 = oagpu_load_genotypes(ctx, buf, g64 ? OAGPU_F64 : OAGPU_F32, OAGPU_HOST, P.n, nf * P.r_file,
+                                        nv, src.data(), P.inter ? env.data() : nullptr, v0);
+    if (r2 != OAGPU_OK) throw Fail(1, oagpu_last_error(ctx));
+  };
+  // Collect a finished scan's hits (the host buffer matches the device
+  // arena, so an overflow is an arena overflow).
+  auto collect = [&](int f0, std::vector<oagpu_hit>& out_hits) -> size_t {
+    oagpu_stats st{};
+    const int r2 = oagpu_scan_wait(ctx, nullptr, hits.data(), (long long)hits.size(), &st);
+    if (r2 == OAGPU_ERR_HIT_OVERFLOW)
+      throw Fail(1, std::string(oagpu_last_error(ctx)) + " -- raise the thresholds (-s/-e) or split the input");
+    if (r2 != OAGPU_OK) throw Fail(1, oagpu_last_error(ctx));
+    total_sig += st.n_sig;
+    out_hits.assign(hits.begin(), hits.begin() + (size_t)st.n_hits);
+    (void)f0;
+    return (size_t)st.n_hits;
+  };
+  // Regroup a block's hits into the reference's (trait block, SNP block) lists.
+  auto write_hits = [&](std::vector<oagpu_hit>& h) {
+    std::stable_sort(h.begin(), h.end(), [&](const oagpu_hit& a, const oagpu_hit& b) {
+      return a.snp / P.mb < b.snp / P.mb;
+    });
+    for (size_t i = 0; i < h.size();) {
+      const int blk = h[i].snp / P.mb;
+      size_t j = i;
+      while (j < h.size() && h[j].snp / P.mb == blk) ++j;
+      writer.write_block(h.data() + i, j - i, blk * P.mb);
+      i = j;
+    }
   };
 
   for (int y0 = 0; y0 < P.t; y0 += P.tb) {
@@ -407,56 +456,32 @@ static int solve_impl(const omicabelnomm_settings* sin, omicabelnomm_outputs* ou
     if (rc != OAGPU_OK) throw Fail(1, oagpu_last_error(ctx));
     acc_loady += now_s() - t0;
 
-    // Genotype blocks, the next one read by a helper thread meanwhile.
-    std::vector<unsigned char> gcur, gnext;
     t0 = now_s();
-    read_geno(0, gcur);
+    read_geno(0, hbuf[0]);
     acc_loadxr += now_s() - t0;
-    for (int f0 = 0; f0 < P.m_file; f0 += gb_file) {
-      const int nf = std::min(gb_file, P.m_file - f0);
-      std::future<void> pre;
-      if (f0 + gb_file < P.m_file)
-        pre = std::async(std::launch::async, [&, f1 = f0 + gb_file] { read_geno(f1, gnext); });
-      std::vector<int32_t> src, env;
-      int v0 = 0, nv = 0;
-      descriptors(P, 0, nf, src, env, v0, nv);
-      v0 = P.multi ? f0 * P.num_inter : f0;
-      t0 = now_s();
-      rc = oagpu_load_genotypes(ctx, gcur.data(), g64 ? OAGPU_F64 : OAGPU_F32, OAGPU_HOST, P.n,
-                                nf * P.r_file, nv, src.data(), P.inter ? env.data() : nullptr, v0);
-      if (rc != OAGPU_OK) throw Fail(1, oagpu_last_error(ctx));
-      oagpu_stats st{};
-      rc = oagpu_scan(ctx, nullptr, hits.data(), (long long)hits.size(), &st);
-      if (rc == OAGPU_ERR_HIT_OVERFLOW && st.n_hits <= gp.max_hits) {
-        // the device arena held them all: rescan into a host buffer that fits
-        hits.resize((size_t)st.n_hits);
-        rc = oagpu_scan(ctx, nullptr, hits.data(), (long long)hits.size(), &st);
-      }
-      if (rc == OAGPU_ERR_HIT_OVERFLOW)
-        throw Fail(1, std::string(oagpu_last_error(ctx)) + " -- raise the thresholds (-s/-e) or split the input");
-      if (rc != OAGPU_OK) throw Fail(1, oagpu_last_error(ctx));
-      acc_gpu += now_s() - t0;
-      total_sig += st.n_sig;
-      // Regroup into the reference's (trait block, SNP block) lists.
-      t0 = now_s();
-      const size_t nh = (size_t)st.n_hits;
-      std::stable_sort(hits.begin(), hits.begin() + nh, [&](const oagpu_hit& a, const oagpu_hit& b) {
-        return a.snp / P.mb < b.snp / P.mb;
-      });
-      for (size_t i = 0; i < nh;) {
-        const int blk = hits[i].snp / P.mb;
-        size_t j = i;
-        while (j < nh && hits[j].snp / P.mb == blk) ++j;
-        writer.write_block(hits.data() + i, j - i, blk * P.mb);
-        i = j;
-      }
-      acc_store += now_s() - t0;
-      if (pre.valid()) {
+    upload(0, hbuf[0]);
+    rc = oagpu_scan_async(ctx, 0);
+    if (rc != OAGPU_OK) throw Fail(1, oagpu_last_error(ctx));
+    std::vector<oagpu_hit> done;
+    int bi = 0;
+    for (int f0 = 0; f0 < P.m_file; f0 += gb_file, ++bi) {
+      const int f1 = f0 + gb_file;
+      if (f1 < P.m_file) {
         t0 = now_s();
-        pre.get();
+        read_geno(f1, hbuf[(bi + 1) & 1]);   // overlaps the scan of block f0
         acc_loadxr += now_s() - t0;
-        std::swap(gcur, gnext);
+        upload(f1, hbuf[(bi + 1) & 1]);      // its copy overlaps that scan too
       }
+      t0 = now_s();
+      collect(f0, done);
+      if (f1 < P.m_file) {
+        rc = oagpu_scan_async(ctx, 0);
+        if (rc != OAGPU_OK) throw Fail(1, oagpu_last_error(ctx));
+      }
+      acc_gpu += now_s() - t0;
+      t0 = now_s();
+      write_hits(done);                      // overlaps the next scan
+      acc_store += now_s() - t0;
     }
   }
   if (out) {

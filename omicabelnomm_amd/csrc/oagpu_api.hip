@@ -53,22 +53,41 @@ struct oagpu_ctx {
   unsigned long long* hash = nullptr;
   int* scratch_i = nullptr;   // rep_of / mismatch
   int G = 0, max_pat_cnt = 0;
-  unsigned long long* rowmask = nullptr;   // per row: patterns it is missing in (G <= 64)
-  bool use_rowmask = false;
+  bool fused_corr = false;   // corrections computed in the operand pass
   int *pat_cnt = nullptr, *pat_rep = nullptr, *pat_pd = nullptr;
   double *UA = nullptr, *UA_invd = nullptr;
   size_t pat_cap = 0;
 
-  // genotypes
+  // genotypes: host blocks are uploaded on a second stream into one of two
+  // device slots, so the next block's copy overlaps the current scan
   bool have_geno = false;
   const void* Gm = nullptr;
   int gdt = 0;
   long long ldg = 0;
   int n_src = 0, m = 0, snp_offset = 0;
-  void* G_own = nullptr;
-  size_t G_own_bytes = 0;
+  hipStream_t cst = nullptr;
+  void* G_slot[2] = {nullptr, nullptr};
+  size_t G_slot_cap[2] = {0, 0};
+  int* d_src[2] = {nullptr, nullptr};
+  int* d_env[2] = {nullptr, nullptr};
+  size_t d_desc_cap[2] = {0, 0};
+  std::vector<int> h_desc[2];
+  hipEvent_t up_ev[2] = {nullptr, nullptr}, free_ev[2] = {nullptr, nullptr};
+  bool free_valid[2] = {false, false};
+  int gslot = 1;
+  bool upload_pending = false;
   int *col_src = nullptr, *col_env = nullptr;
-  size_t desc_cap = 0;
+
+  // a scan in flight (oagpu_scan_async .. oagpu_scan_wait)
+  bool pending = false;
+  bool p_dense = false;
+  int p_nchunks = 0, p_t = 0, p_m = 0;
+  std::vector<hipEvent_t> pev;
+  hipEvent_t done_ev = nullptr;
+  unsigned long long* cnt_h = nullptr;   // pinned copy of the counters
+  double *pd_beta = nullptr, *pd_se = nullptr, *pd_t = nullptr, *pd_p = nullptr, *pd_r2 = nullptr,
+         *pd_rss = nullptr, *pd_varx = nullptr;
+  uint8_t *pd_kept = nullptr, *pd_em = nullptr;
 
   // chunk buffers
   int mc_cap = 0;
@@ -250,6 +269,13 @@ extern "C" int oagpu_create(const oagpu_params* prm, oagpu_ctx** out) {
   if (prm->device < 0 || prm->device >= ndev) return fail(c, OAGPU_ERR_ARG, "bad device");
   HIPCK(c, hipSetDevice(prm->device));
   HIPCK(c, hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+  HIPCK(c, hipStreamCreateWithFlags(&c->cst, hipStreamNonBlocking));
+  for (int i = 0; i < 2; ++i) {
+    HIPCK(c, hipEventCreateWithFlags(&c->up_ev[i], hipEventDisableTiming));
+    HIPCK(c, hipEventCreateWithFlags(&c->free_ev[i], hipEventDisableTiming));
+  }
+  HIPCK(c, hipEventCreate(&c->done_ev));
+  HIPCK(c, hipHostMalloc((void**)&c->cnt_h, sizeof(unsigned long long) * CNT_N, hipHostMallocDefault));
   c->n = n; c->l = l; c->r = r; c->p = l + r;
   c->Q = l * r + r * (r + 1) / 2;
   c->n_pad = (n + 63) / 64 * 64;
@@ -282,20 +308,34 @@ extern "C" int oagpu_create(const oagpu_params* prm, oagpu_ctx** out) {
   HIPCK(c, hipMalloc(&c->rt, sizeof(double) * (size_t)l * c->ldt));
   HIPCK(c, hipMalloc(&c->zt, sizeof(double) * (size_t)l * c->ldt));
   HIPCK(c, hipMalloc(&c->hash, sizeof(unsigned long long) * 2 * T));
-  HIPCK(c, hipMalloc(&c->rowmask, sizeof(unsigned long long) * (size_t)n));
   return OAGPU_OK;
 }
 
 extern "C" void oagpu_destroy(oagpu_ctx* c) {
   if (!c) return;
   if (c->st) (void)hipStreamSynchronize(c->st);
+  if (c->cst) (void)hipStreamSynchronize(c->cst);
+  for (int i = 0; i < 2; ++i) {
+    dfree(c->G_slot[i]);
+    dfree(c->d_src[i]);
+    dfree(c->d_env[i]);
+    if (c->up_ev[i]) (void)hipEventDestroy(c->up_ev[i]);
+    if (c->free_ev[i]) (void)hipEventDestroy(c->free_ev[i]);
+  }
+  for (auto& e : c->pev) (void)hipEventDestroy(e);
+  if (c->done_ev) (void)hipEventDestroy(c->done_ev);
+  if (c->cnt_h) (void)hipHostFree(c->cnt_h);
+  void* dense_bufs[] = {c->pd_beta, c->pd_se, c->pd_t, c->pd_p, c->pd_kept, c->pd_r2, c->pd_rss, c->pd_em,
+                        c->pd_varx};
+  for (void* b : dense_bufs) dfree(b);
   void* bufs[] = {c->XL_cm, c->XL_rm, c->U0, c->Stl_inv_d, c->Stl_pk, c->XLop_hi, c->XLop_lo, c->E,
                   c->Y_own, c->Bhi, c->Blo, c->rows, c->ymiss, c->yzero, c->ypat, c->yy, c->sst,
                   c->rss0, c->rt, c->zt, c->hash, c->scratch_i, c->pat_cnt, c->pat_rep, c->pat_pd,
-                  c->UA, c->UA_invd, c->rowmask, c->G_own, c->col_src, c->col_env, c->Ahi, c->Alo, c->Cbuf,
+                  c->UA, c->UA_invd, c->Ahi, c->Alo, c->Cbuf,
                   c->col_mean, c->Sbr, c->Str, c->varx, c->Ccorr, c->hits, c->counters, c->fb};
   for (void* b : bufs) dfree(b);
   if (c->st) (void)hipStreamDestroy(c->st);
+  if (c->cst) (void)hipStreamDestroy(c->cst);
   delete c;
 }
 
@@ -500,12 +540,13 @@ extern "C" int oagpu_load_traits(oagpu_ctx* c, const void* Y, int dtype, int whe
   d = make_dev(c);
   launch_pattern(d, c->Stl_pk, G, c->pat_rep, c->UA, c->UA_invd, c->pat_pd, c->st);
   HIPCK(c, hipGetLastError());
-  // Few patterns: corrections are accumulated inside the operand pass.
-  c->use_rowmask = c->max_pat_cnt > 0 && G <= 64 && prep_corr_lds_bytes(G, c->Q) <= 65536;
-  if (c->use_rowmask) {
-    HIPCK(c, hipMemsetAsync(c->rowmask, 0, sizeof(unsigned long long) * (size_t)n, c->st));
-    launch_rowmask(G, n, c->pat_cnt, c->pat_rep, c->rows, c->rowmask, c->st);
-    HIPCK(c, hipGetLastError());
+  // Few missing rows in total: the corrections are computed in the operand
+  // pass; many patterns (e.g. independent missingness per trait) use the
+  // separate SNP-tiled kernel.
+  {
+    long long tot = 0;
+    for (int g = 0; g < G; ++g) tot += pat_cnt[g];
+    c->fused_corr = c->max_pat_cnt > 0 && tot <= 32768;
   }
   d = make_dev(c);
   launch_trait_solve(d, c->lb, c->zt, c->rss0, c->st);
@@ -531,6 +572,9 @@ extern "C" int oagpu_load_genotypes(oagpu_ctx* c, const void* G, int dtype, int 
     for (long long i = 0; i < (long long)m * r; ++i)
       if (col_env[i] >= c->n_env) return fail(c, OAGPU_ERR_ARG, "interaction column out of range");
   }
+  // The other slot; wait until the scan that last read it has finished.
+  const int slot = c->gslot ^ 1;
+  if (c->free_valid[slot]) HIPCK(c, hipStreamWaitEvent(c->cst, c->free_ev[slot], 0));
   c->gdt = dtype;
   c->ldg = ld;
   c->n_src = n_src;
@@ -538,39 +582,55 @@ extern "C" int oagpu_load_genotypes(oagpu_ctx* c, const void* G, int dtype, int 
   c->snp_offset = snp_offset;
   if (where == OAGPU_HOST) {
     const size_t bytes = dt_size(dtype) * ((size_t)ld * (n_src - 1) + c->n);
-    HIPCK(c, grow(&c->G_own, &c->G_own_bytes, bytes));
-    HIPCK(c, hipMemcpyAsync(c->G_own, G, bytes, hipMemcpyHostToDevice, c->st));
-    c->Gm = c->G_own;
+    if (c->G_slot_cap[slot] < bytes) {
+      HIPCK(c, hipStreamSynchronize(c->cst));
+      HIPCK(c, grow(&c->G_slot[slot], &c->G_slot_cap[slot], bytes));
+    }
+    HIPCK(c, hipMemcpyAsync(c->G_slot[slot], G, bytes, hipMemcpyHostToDevice, c->cst));
+    c->Gm = c->G_slot[slot];
   } else {
     c->Gm = G;
   }
   const size_t nd = (size_t)m * r;
   if (col_src || col_env) {
-    if (nd > c->desc_cap) {
-      dfree(c->col_src); dfree(c->col_env);
-      c->col_src = c->col_env = nullptr;
-      HIPCK(c, hipMalloc(&c->col_src, sizeof(int) * nd));
-      HIPCK(c, hipMalloc(&c->col_env, sizeof(int) * nd));
-      c->desc_cap = nd;
+    if (nd > c->d_desc_cap[slot]) {
+      HIPCK(c, hipStreamSynchronize(c->cst));
+      dfree(c->d_src[slot]);
+      dfree(c->d_env[slot]);
+      c->d_src[slot] = c->d_env[slot] = nullptr;
+      HIPCK(c, hipMalloc(&c->d_src[slot], sizeof(int) * nd));
+      HIPCK(c, hipMalloc(&c->d_env[slot], sizeof(int) * nd));
+      c->d_desc_cap[slot] = nd;
     }
-    std::vector<int> src(nd), env(nd);
+    std::vector<int>& h = c->h_desc[slot];
+    h.resize(2 * nd);
     for (size_t i = 0; i < nd; ++i) {
-      src[i] = col_src ? col_src[i] : (int)i;
-      env[i] = col_env ? col_env[i] : -1;
+      h[i] = col_src ? col_src[i] : (int)i;
+      h[nd + i] = col_env ? col_env[i] : -1;
     }
-    HIPCK(c, hipMemcpyAsync(c->col_src, src.data(), sizeof(int) * nd, hipMemcpyHostToDevice, c->st));
-    HIPCK(c, hipMemcpyAsync(c->col_env, env.data(), sizeof(int) * nd, hipMemcpyHostToDevice, c->st));
-    HIPCK(c, hipStreamSynchronize(c->st));
-    c->have_geno = true;
-    return OAGPU_OK;
+    HIPCK(c, hipMemcpyAsync(c->d_src[slot], h.data(), sizeof(int) * nd, hipMemcpyHostToDevice, c->cst));
+    HIPCK(c, hipMemcpyAsync(c->d_env[slot], h.data() + nd, sizeof(int) * nd, hipMemcpyHostToDevice, c->cst));
+    c->col_src = c->d_src[slot];
+    c->col_env = c->d_env[slot];
+  } else {
+    // identity descriptors: kernels read col_src == nullptr as s*r + c
+    c->col_src = c->col_env = nullptr;
   }
-  // Identity descriptors: kernels read col_src == nullptr as s*r + c.
-  dfree(c->col_src); dfree(c->col_env);
-  c->col_src = c->col_env = nullptr;
-  c->desc_cap = 0;
-  HIPCK(c, hipStreamSynchronize(c->st));
+  HIPCK(c, hipEventRecord(c->up_ev[slot], c->cst));
+  c->gslot = slot;
+  c->upload_pending = true;
   c->have_geno = true;
   return OAGPU_OK;
+}
+
+extern "C" int oagpu_host_alloc(size_t bytes, void** ptr) {
+  if (!ptr) return OAGPU_ERR_ARG;
+  *ptr = nullptr;
+  return hipHostMalloc(ptr, bytes > 0 ? bytes : 64, hipHostMallocDefault) == hipSuccess ? OAGPU_OK : OAGPU_ERR_HIP;
+}
+
+extern "C" void oagpu_host_free(void* ptr) {
+  if (ptr) (void)hipHostFree(ptr);
 }
 
 static int ensure_chunk(oagpu_ctx* c, int mc) {
@@ -595,9 +655,8 @@ static int ensure_chunk(oagpu_ctx* c, int mc) {
   return OAGPU_OK;
 }
 
-extern "C" int oagpu_scan(oagpu_ctx* c, oagpu_dense* dense, oagpu_hit* hits, long long hit_cap,
-                          oagpu_stats* stats) {
-  if (!c) return OAGPU_ERR_ARG;
+static int scan_enqueue(oagpu_ctx* c, bool want_dense) {
+  if (c->pending) return fail(c, OAGPU_ERR_STATE, "a scan is already in flight (oagpu_scan_wait first)");
   if (!c->have_traits || !c->have_geno) return fail(c, OAGPU_ERR_STATE, "load traits and genotypes before scanning");
   HIPCK(c, hipSetDevice(c->prm.device));
   const int r = c->r, p = c->p, t = c->t, m = c->m, l = c->l;
@@ -611,48 +670,47 @@ extern "C" int oagpu_scan(oagpu_ctx* c, oagpu_dense* dense, oagpu_hit* hits, lon
   int rc = ensure_chunk(c, (int)mc);
   if (rc) return rc;
 
-  // Dense outputs on the device when requested.
-  double *d_beta = nullptr, *d_se = nullptr, *d_t = nullptr, *d_p = nullptr, *d_r2 = nullptr, *d_rss = nullptr,
-         *d_varx = nullptr;
-  uint8_t *d_kept = nullptr, *d_em = nullptr;
   const size_t npair = (size_t)t * m;
-  const bool want_dense = dense && (dense->beta || dense->se || dense->tstat || dense->pval || dense->kept ||
-                                    dense->r2 || dense->rss || dense->emitted || dense->var_x || dense->n_used);
   if (want_dense) {
-    HIPCK(c, hipMalloc(&d_beta, sizeof(double) * npair * p));
-    HIPCK(c, hipMalloc(&d_se, sizeof(double) * npair * p));
-    HIPCK(c, hipMalloc(&d_t, sizeof(double) * npair * p));
-    HIPCK(c, hipMalloc(&d_p, sizeof(double) * npair * p));
-    HIPCK(c, hipMalloc(&d_kept, npair * p));
-    HIPCK(c, hipMalloc(&d_r2, sizeof(double) * npair));
-    HIPCK(c, hipMalloc(&d_rss, sizeof(double) * npair));
-    HIPCK(c, hipMalloc(&d_em, npair));
-    HIPCK(c, hipMalloc(&d_varx, sizeof(double) * (size_t)m * p));
+    HIPCK(c, hipMalloc(&c->pd_beta, sizeof(double) * npair * p));
+    HIPCK(c, hipMalloc(&c->pd_se, sizeof(double) * npair * p));
+    HIPCK(c, hipMalloc(&c->pd_t, sizeof(double) * npair * p));
+    HIPCK(c, hipMalloc(&c->pd_p, sizeof(double) * npair * p));
+    HIPCK(c, hipMalloc(&c->pd_kept, npair * p));
+    HIPCK(c, hipMalloc(&c->pd_r2, sizeof(double) * npair));
+    HIPCK(c, hipMalloc(&c->pd_rss, sizeof(double) * npair));
+    HIPCK(c, hipMalloc(&c->pd_em, npair));
+    HIPCK(c, hipMalloc(&c->pd_varx, sizeof(double) * (size_t)m * p));
+  }
+  if (c->upload_pending) {
+    HIPCK(c, hipStreamWaitEvent(c->st, c->up_ev[c->gslot], 0));
+    c->upload_pending = false;
   }
   for (int i = 0; i < CNT_N; ++i) launch_reset_counter(c->counters, i, c->st);
 
   const int nchunks = (int)((m + mc - 1) / mc);
-  std::vector<hipEvent_t> ev((size_t)nchunks * 7);
-  for (auto& e : ev) HIPCK(c, hipEventCreate(&e));
+  for (auto& e : c->pev) (void)hipEventDestroy(e);
+  c->pev.assign((size_t)nchunks * 7, nullptr);
+  for (auto& e : c->pev) HIPCK(c, hipEventCreate(&e));
   DevScan d = make_dev(c);
   d.need_bl = c->prm.disp_cov || want_dense;
-  d.d_beta = d_beta; d.d_se = d_se; d.d_t = d_t; d.d_p = d_p; d.d_kept = d_kept;
-  d.d_r2 = d_r2; d.d_rss = d_rss; d.d_emitted = d_em; d.d_varx = d_varx;
+  d.d_beta = c->pd_beta; d.d_se = c->pd_se; d.d_t = c->pd_t; d.d_p = c->pd_p; d.d_kept = c->pd_kept;
+  d.d_r2 = c->pd_r2; d.d_rss = c->pd_rss; d.d_emitted = c->pd_em; d.d_varx = c->pd_varx;
   for (int ci = 0; ci < nchunks; ++ci) {
     const int s0 = (int)(ci * mc);
     const int mcc = (int)std::min<long long>(mc, m - s0);
     d.s0 = s0;
     d.mc = mcc;
-    hipEvent_t* E = &ev[(size_t)ci * 7];
+    hipEvent_t* E = &c->pev[(size_t)ci * 7];
     HIPCK(c, hipEventRecord(E[0], c->st));
-    launch_prep(d, c->prm.precision, c->Ahi, c->Alo, c->use_rowmask ? c->rowmask : nullptr, c->st);
+    launch_prep(d, c->prm.precision, c->Ahi, c->Alo, c->fused_corr, c->st);
     HIPCK(c, hipEventRecord(E[1], c->st));
     launch_gemm(c->prm.precision, c->Ahi, c->Alo, c->Bhi, c->Blo, c->Cbuf, mcc * r, t + l, c->n_pad, t + l,
                 c->st);
     HIPCK(c, hipEventRecord(E[2], c->st));
     launch_snp_gram(d, c->prm.precision, c->lb, c->st);
     HIPCK(c, hipEventRecord(E[3], c->st));
-    if (c->max_pat_cnt > 0 && !c->use_rowmask) launch_mask(d, c->lb, c->st);
+    if (c->max_pat_cnt > 0 && !c->fused_corr) launch_mask(d, c->lb, c->st);
     HIPCK(c, hipEventRecord(E[4], c->st));
     launch_reset_counter(c->counters, CNT_FB_CHUNK, c->st);
     launch_epilogue(d, c->prm.precision, c->lb, c->st);
@@ -662,12 +720,28 @@ extern "C" int oagpu_scan(oagpu_ctx* c, oagpu_dense* dense, oagpu_hit* hits, lon
     HIPCK(c, hipGetLastError());
   }
   c->gemm_variant = gemm_variant_name(c->prm.precision, (int)std::min<long long>(mc, m) * r, t + l, c->n_pad);
-  unsigned long long cnt[CNT_N];
-  HIPCK(c, hipMemcpyAsync(cnt, c->counters, sizeof(cnt), hipMemcpyDeviceToHost, c->st));
-  HIPCK(c, hipStreamSynchronize(c->st));
+  // the genotype slot may be overwritten once everything above has run
+  HIPCK(c, hipEventRecord(c->free_ev[c->gslot], c->st));
+  c->free_valid[c->gslot] = true;
+  HIPCK(c, hipMemcpyAsync(c->cnt_h, c->counters, sizeof(unsigned long long) * CNT_N, hipMemcpyDeviceToHost, c->st));
+  HIPCK(c, hipEventRecord(c->done_ev, c->st));
+  c->pending = true;
+  c->p_dense = want_dense;
+  c->p_nchunks = nchunks;
+  c->p_t = t;
+  c->p_m = m;
+  return OAGPU_OK;
+}
+
+static int scan_collect(oagpu_ctx* c, oagpu_dense* dense, oagpu_hit* hits, long long hit_cap,
+                        oagpu_stats* stats) {
+  if (!c->pending) return fail(c, OAGPU_ERR_STATE, "no scan in flight");
+  c->pending = false;
+  HIPCK(c, hipEventSynchronize(c->done_ev));
+  const int p = c->p, t = c->p_t, m = c->p_m, nchunks = c->p_nchunks;
   for (int i = 0; i < 8; ++i) c->timings[i] = 0.0;
   for (int ci = 0; ci < nchunks; ++ci) {
-    hipEvent_t* E = &ev[(size_t)ci * 7];
+    hipEvent_t* E = &c->pev[(size_t)ci * 7];
     for (int k = 0; k < 6; ++k) {
       float ms = 0.f;
       HIPCK(c, hipEventElapsedTime(&ms, E[k], E[k + 1]));
@@ -676,11 +750,11 @@ extern "C" int oagpu_scan(oagpu_ctx* c, oagpu_dense* dense, oagpu_hit* hits, lon
   }
   {
     float ms = 0.f;
-    HIPCK(c, hipEventElapsedTime(&ms, ev[0], ev[(size_t)(nchunks - 1) * 7 + 6]));
+    HIPCK(c, hipEventElapsedTime(&ms, c->pev[0], c->pev[(size_t)(nchunks - 1) * 7 + 6]));
     c->timings[6] = ms;
   }
-  for (auto& e : ev) (void)hipEventDestroy(e);
-
+  unsigned long long cnt[CNT_N];
+  std::memcpy(cnt, c->cnt_h, sizeof(cnt));
   const long long nh = (long long)cnt[CNT_HITS];
   if (stats) {
     stats->n_hits = nh;
@@ -694,25 +768,30 @@ extern "C" int oagpu_scan(oagpu_ctx* c, oagpu_dense* dense, oagpu_hit* hits, lon
   int status = OAGPU_OK;
   if (cnt[CNT_ERR] & 1ull)
     status = fail(c, OAGPU_ERR_NUMERIC, "Error with Syy < 0 or var_x < 0 (the reference exits here)");
-  if (want_dense) {
+  if (c->p_dense) {
+    const size_t npair = (size_t)t * m;
     auto cp = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
-      if (!dst) return hipSuccess;
+      if (!dst || !dense) return hipSuccess;
       return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
     };
-    HIPCK(c, cp(dense->beta, d_beta, sizeof(double) * npair * p));
-    HIPCK(c, cp(dense->se, d_se, sizeof(double) * npair * p));
-    HIPCK(c, cp(dense->tstat, d_t, sizeof(double) * npair * p));
-    HIPCK(c, cp(dense->pval, d_p, sizeof(double) * npair * p));
-    HIPCK(c, cp(dense->kept, d_kept, npair * p));
-    HIPCK(c, cp(dense->r2, d_r2, sizeof(double) * npair));
-    HIPCK(c, cp(dense->rss, d_rss, sizeof(double) * npair));
-    HIPCK(c, cp(dense->emitted, d_em, npair));
-    HIPCK(c, cp(dense->var_x, d_varx, sizeof(double) * (size_t)m * p));
-    HIPCK(c, cp(dense->n_used, c->ymiss, sizeof(int) * t));
-    if (dense->n_used)
-      for (int y = 0; y < t; ++y) dense->n_used[y] = c->n - dense->n_used[y];
-    void* bufs[] = {d_beta, d_se, d_t, d_p, d_kept, d_r2, d_rss, d_em, d_varx};
+    if (dense) {
+      HIPCK(c, cp(dense->beta, c->pd_beta, sizeof(double) * npair * p));
+      HIPCK(c, cp(dense->se, c->pd_se, sizeof(double) * npair * p));
+      HIPCK(c, cp(dense->tstat, c->pd_t, sizeof(double) * npair * p));
+      HIPCK(c, cp(dense->pval, c->pd_p, sizeof(double) * npair * p));
+      HIPCK(c, cp(dense->kept, c->pd_kept, npair * p));
+      HIPCK(c, cp(dense->r2, c->pd_r2, sizeof(double) * npair));
+      HIPCK(c, cp(dense->rss, c->pd_rss, sizeof(double) * npair));
+      HIPCK(c, cp(dense->emitted, c->pd_em, npair));
+      HIPCK(c, cp(dense->var_x, c->pd_varx, sizeof(double) * (size_t)m * p));
+      HIPCK(c, cp(dense->n_used, c->ymiss, sizeof(int) * t));
+      if (dense->n_used)
+        for (int y = 0; y < t; ++y) dense->n_used[y] = c->n - dense->n_used[y];
+    }
+    void* bufs[] = {c->pd_beta, c->pd_se, c->pd_t, c->pd_p, c->pd_kept, c->pd_r2, c->pd_rss, c->pd_em, c->pd_varx};
     for (void* b : bufs) dfree(b);
+    c->pd_beta = c->pd_se = c->pd_t = c->pd_p = c->pd_r2 = c->pd_rss = c->pd_varx = nullptr;
+    c->pd_kept = c->pd_em = nullptr;
   }
   if (nh > c->hit_cap || (hits && nh > hit_cap)) {
     return fail(c, OAGPU_ERR_HIT_OVERFLOW,
@@ -728,6 +807,30 @@ extern "C" int oagpu_scan(oagpu_ctx* c, oagpu_dense* dense, oagpu_hit* hits, lon
     });
   }
   return status;
+}
+
+static bool dense_requested(const oagpu_dense* dense) {
+  return dense && (dense->beta || dense->se || dense->tstat || dense->pval || dense->kept || dense->r2 ||
+                   dense->rss || dense->emitted || dense->var_x || dense->n_used);
+}
+
+extern "C" int oagpu_scan(oagpu_ctx* c, oagpu_dense* dense, oagpu_hit* hits, long long hit_cap,
+                          oagpu_stats* stats) {
+  if (!c) return OAGPU_ERR_ARG;
+  const int rc = scan_enqueue(c, dense_requested(dense));
+  if (rc != OAGPU_OK) return rc;
+  return scan_collect(c, dense, hits, hit_cap, stats);
+}
+
+extern "C" int oagpu_scan_async(oagpu_ctx* c, int want_dense) {
+  if (!c) return OAGPU_ERR_ARG;
+  return scan_enqueue(c, want_dense != 0);
+}
+
+extern "C" int oagpu_scan_wait(oagpu_ctx* c, oagpu_dense* dense, oagpu_hit* hits, long long hit_cap,
+                               oagpu_stats* stats) {
+  if (!c) return OAGPU_ERR_ARG;
+  return scan_collect(c, dense, hits, hit_cap, stats);
 }
 
 extern "C" int oagpu_last_timings(const oagpu_ctx* c, double* out, int n) {

@@ -413,6 +413,114 @@ __global__ __launch_bounds__(256, 1) void k_gemm_bf16x3_big(const uint16_t* __re
     }
 }
 
+
+// 128 x 128 split-BF16 tiles with the global loads issued two k-steps
+// ahead of their use (two register stages), so ~1500 cycles of MFMA work
+// cover the memory latency instead of ~770.
+__global__ __launch_bounds__(256, 2) void k_gemm_bf16x3_pf2(const uint16_t* __restrict__ Ah,
+                                                           const uint16_t* __restrict__ Al,
+                                                           const uint16_t* __restrict__ Bh,
+                                                           const uint16_t* __restrict__ Bl,
+                                                           float* __restrict__ C, int M, int N, int K,
+                                                           int ldc, int tiles_n, int total) {
+  __shared__ __attribute__((aligned(16))) uint16_t sm[2][4][128 * LDS_ROW];   // [stage][Ah,Al,Bh,Bl]
+  const int tile = xcd_tile(blockIdx.x, total);
+  const int tm = tile / tiles_n, tn = tile - tm * tiles_n;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
+  const int i = lane & 15, g = lane >> 4;
+  const int wr = (w & 1) * 64, wc = (w >> 1) * 64;
+  size_t goff[2][2];
+  int soff[2];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int c = tid + q * 256, row = c >> 2, ch = c & 3;
+    goff[q][0] = (size_t)(tm * 128 + row) * K + 8 * ch;
+    goff[q][1] = (size_t)(tn * 128 + row) * K + 8 * ch;
+    soff[q] = row * LDS_ROW + 8 * ch;
+  }
+  u4 ra[2][4], rb[2][4];   // two register stages
+  auto gload = [&](u4 (&st)[2][4], int k0) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      st[q][0] = *reinterpret_cast<const u4*>(Ah + goff[q][0] + k0);
+      st[q][1] = *reinterpret_cast<const u4*>(Al + goff[q][0] + k0);
+      st[q][2] = *reinterpret_cast<const u4*>(Bh + goff[q][1] + k0);
+      st[q][3] = *reinterpret_cast<const u4*>(Bl + goff[q][1] + k0);
+    }
+  };
+  auto sstore = [&](const u4 (&st)[2][4], int stage) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int a = 0; a < 4; ++a) *reinterpret_cast<u4*>(&sm[stage][a][soff[q]]) = st[q][a];
+  };
+  f4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+
+  gload(ra, 0);
+  if (K > 32) gload(rb, 32);
+  sstore(ra, 0);
+  __syncthreads();
+  int cur = 0;
+  for (int k0 = 0; k0 < K; k0 += 64) {
+    // step k0 (stage cur, data of ra already in LDS); rb holds k0 + 32
+    if (k0 + 64 < K) gload(ra, k0 + 64);
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      if (half == 1 && k0 + 32 >= K) break;
+      u4 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const int rA = (wr + 16 * a + i) * LDS_ROW + 8 * g;
+        const int rB = (wc + 16 * a + i) * LDS_ROW + 8 * g;
+        ah[a] = *reinterpret_cast<const u4*>(&sm[cur][0][rA]);
+        al[a] = *reinterpret_cast<const u4*>(&sm[cur][1][rA]);
+        bh[a] = *reinterpret_cast<const u4*>(&sm[cur][2][rB]);
+        bl[a] = *reinterpret_cast<const u4*>(&sm[cur][3][rB]);
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(al[a]), as_bf16x8(bh[b]), acc[a][b], 0, 0, 0);
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(ah[a]), as_bf16x8(bl[b]), acc[a][b], 0, 0, 0);
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(ah[a]), as_bf16x8(bh[b]), acc[a][b], 0, 0, 0);
+      // stage the next k-step: rb (k0+32) after the first half, ra (k0+64) after the second
+      if (half == 0) {
+        if (k0 + 32 < K) sstore(rb, cur ^ 1);
+      } else {
+        if (k0 + 64 < K) sstore(ra, cur ^ 1);
+      }
+      __syncthreads();
+      cur ^= 1;
+      if (half == 0 && k0 + 96 < K) gload(rb, k0 + 96);
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int col = tn * 128 + wc + 16 * b + i;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = tm * 128 + wr + 16 * a + 4 * g + v;
+        if (row < M && col < N) C[(size_t)row * ldc + col] = acc[a][b][v];
+      }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Kernel variants and the autotuner.  Each precision has a list of
 // variants; the first launch for a (precision, M, N, K) shape class times
@@ -450,6 +558,13 @@ static void v_bf_lds(const void* Ah, const void* Al, const void* Bh, const void*
                      (const uint16_t*)Al, (const uint16_t*)Bh, (const uint16_t*)Bl, (float*)C, M, N, K, ldc,
                      tiles_n, total);
 }
+static void v_bf_pf2(const void* Ah, const void* Al, const void* Bh, const void* Bl, void* C, int M, int N,
+                     int K, int ldc, hipStream_t st) {
+  const int tiles_m = (M + 127) / 128, tiles_n = (N + 127) / 128, total = tiles_m * tiles_n;
+  hipLaunchKernelGGL(k_gemm_bf16x3_pf2, dim3(total), dim3(256), 0, st, (const uint16_t*)Ah,
+                     (const uint16_t*)Al, (const uint16_t*)Bh, (const uint16_t*)Bl, (float*)C, M, N, K, ldc,
+                     tiles_n, total);
+}
 static void v_bf_big(const void* Ah, const void* Al, const void* Bh, const void* Bl, void* C, int M, int N,
                      int K, int ldc, hipStream_t st) {
   const int tiles_m = (M + 255) / 256, tiles_n = (N + 127) / 128, total = tiles_m * tiles_n;
@@ -461,6 +576,7 @@ static void v_bf_big(const void* Ah, const void* Al, const void* Bh, const void*
 static const GemmVariant kVariants[] = {
     {"f64_direct_128x128", PREC_FP64, 128, 128, v_f64},
     {"f32_direct_128x128", PREC_FP32, 128, 128, v_f32},
+    {"bf16x3_lds_pf2_128x128", PREC_BF16X3, 128, 128, v_bf_pf2},
     {"bf16x3_lds_256x128", PREC_BF16X3, 256, 128, v_bf_big},
     {"bf16x3_lds_128x128", PREC_BF16X3, 128, 128, v_bf_lds},
     {"bf16x3_direct_128x128", PREC_BF16X3, 128, 128, v_bf_direct},

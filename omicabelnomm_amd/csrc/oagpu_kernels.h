@@ -15,13 +15,9 @@ void launch_pattern(const DevScan& d, const double* Stl_pk, int G, const int* pa
 void launch_trait_solve(const DevScan& d, int lb, double* zt, double* rss0, hipStream_t st);
 
 // SNP side (oagpu_kernels_snp.hip)
-// rowmask (nullptr = no fused corrections): bit g of rowmask[k] is set when
-// row k is missing in pattern g (G <= 64).
-void launch_prep(const DevScan& d, int prec, void* Ahi, void* Alo, const unsigned long long* rowmask,
-                 hipStream_t st);
-size_t prep_corr_lds_bytes(int G, int Q);   // LDS the fused corrections need
-void launch_rowmask(int G, int n, const int* pat_cnt, const int* pat_rep, const int* rows,
-                    unsigned long long* mask, hipStream_t st);
+// corr: also compute the missing-row corrections of every pattern (used
+// when the patterns' missing rows are few).
+void launch_prep(const DevScan& d, int prec, void* Ahi, void* Alo, bool corr, hipStream_t st);
 void launch_snp_gram(const DevScan& d, int prec, int lb, hipStream_t st);
 void launch_mask(const DevScan& d, int lb, hipStream_t st);
 void launch_epilogue(const DevScan& d, int prec, int lb, hipStream_t st);

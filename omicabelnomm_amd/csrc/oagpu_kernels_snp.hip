@@ -37,12 +37,9 @@ __device__ __forceinline__ double geno_value(const DevScan& d, int src, int env,
 // ---------------------------------------------------------------------------
 // Operand preparation, SPW SNPs per block: mean imputation, interaction
 // columns, the contraction operand A (column s*R+c, rows n..n_pad-1 zero),
-// Sbr = XR_s'XR_s and -- when the trait block has at most 64 missing-value
-// patterns -- the missing-row corrections C_lr, C_rr of every pattern,
-// accumulated while the columns stream by (rows carry a bitmask of the
-// patterns they are missing in; the sums go to LDS with FP64 atomics).
-// Each thread handles runs of 4 consecutive rows, so loads and operand
-// stores are vectors.
+// Sbr = XR_s'XR_s and -- when the patterns' missing rows are few -- the
+// missing-row corrections C_lr, C_rr of every pattern.  Each thread handles
+// runs of 4 consecutive rows, so loads and operand stores are vectors.
 constexpr int PREP_SPW = 4;
 
 template <int OP>
@@ -65,14 +62,12 @@ __device__ __forceinline__ void store_op4(void* Ahi, void* Alo, size_t o, const 
 }
 
 template <int R, int OP, bool CORR>
-__global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo,
-                                              const unsigned long long* rowmask) {
+__global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo) {
   constexpr int RR = R * (R + 1) / 2;
   constexpr int NV = PREP_SPW * R;
   const int tid = threadIdx.x;
   const int sb = blockIdx.x * PREP_SPW;
   __shared__ double shd[4];
-  extern __shared__ double cacc[];   // [G][PREP_SPW][Q] when CORR
   int src[NV], env[NV];
   bool live[PREP_SPW];
 #pragma unroll
@@ -82,9 +77,6 @@ __global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo,
     col_desc<R>(d, gs, src + q * R, env + q * R);
   }
   const int l = d.l, Q = d.Q, G = d.G;
-  if (CORR) {
-    for (int e = tid; e < G * PREP_SPW * Q; e += 256) cacc[e] = 0.0;
-  }
   // pass 1: NaN counts and sums per column
   double sum[NV], cnt[NV];
 #pragma unroll
@@ -137,33 +129,6 @@ __global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo,
 #pragma unroll
           for (int a = 0; a <= b; ++a) acc[q * RR + upk(a, b)] += x[q * R + a][e] * x[q * R + b][e];
     }
-    if (CORR) {
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int k = k0 + e;
-        unsigned long long m = (k < d.n) ? rowmask[k] : 0ull;
-        if (!m) continue;
-        const double* xl = d.XL_rm + (size_t)k * l;
-        while (m) {
-          const int g = __builtin_ctzll(m);
-          m &= m - 1;
-#pragma unroll
-          for (int q = 0; q < PREP_SPW; ++q) {
-            double* dst = cacc + (size_t)(g * PREP_SPW + q) * Q;
-            for (int j = 0; j < l; ++j) {
-              const double xj = xl[j];
-#pragma unroll
-              for (int c = 0; c < R; ++c) atomicAdd(dst + j * R + c, xj * x[q * R + c][e]);
-            }
-#pragma unroll
-            for (int b = 0; b < R; ++b)
-#pragma unroll
-              for (int a = 0; a <= b; ++a)
-                atomicAdd(dst + l * R + upk(a, b), x[q * R + a][e] * x[q * R + b][e]);
-          }
-        }
-      }
-    }
   }
 #pragma unroll
   for (int e = 0; e < PREP_SPW * RR; ++e) {
@@ -179,11 +144,49 @@ __global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo,
         for (int c = 0; c < R; ++c) d.col_mean[(size_t)(sb + q) * R + c] = mean[q * R + c];
   }
   if (CORR) {
-    __syncthreads();
-    // Ccorr layout [snp][term][pattern]
-    for (int e = tid; e < PREP_SPW * Q * G; e += 256) {
-      const int g = e % G, rest = e / G, qq = rest % Q, q = rest / Q;
-      if (live[q]) d.Ccorr[((size_t)(sb + q) * Q + qq) * G + g] = cacc[(g * PREP_SPW + q) * Q + qq];
+    // Missing-row corrections of every pattern, while the block's columns
+    // are still in cache: one wave per pattern, lanes over its rows, a wave
+    // reduction per term.  Ccorr layout [snp][term][pattern].
+    const int lane = tid & 63, wv = tid >> 6;
+    for (int g = wv; g < G; g += 4) {
+      const int cnt = d.pat_cnt[g];
+      const int* rows = d.rows + (size_t)d.pat_rep[g] * d.n;
+      for (int q = 0; q < PREP_SPW; ++q) {
+        if (!live[q]) continue;
+        const size_t base = (size_t)(sb + q) * Q;
+        for (int j = 0; j <= l; ++j) {
+          double a[RR];
+#pragma unroll
+          for (int e = 0; e < RR; ++e) a[e] = 0.0;
+          for (int i = lane; i < cnt; i += 64) {
+            const int k = rows[i];
+            double xv[R];
+#pragma unroll
+            for (int c = 0; c < R; ++c) {
+              double x = load_elem(d.Gm, d.gdt, (size_t)src[q * R + c] * d.ldg + k);
+              if (env[q * R + c] >= 0) x *= d.E[(size_t)env[q * R + c] * d.lde + k];
+              xv[c] = (x == x) ? x : mean[q * R + c];
+            }
+            if (j < l) {
+              const double xl = d.XL_rm[(size_t)k * l + j];
+#pragma unroll
+              for (int c = 0; c < R; ++c) a[c] += xl * xv[c];
+            } else {
+#pragma unroll
+              for (int b2 = 0; b2 < R; ++b2)
+#pragma unroll
+                for (int a2 = 0; a2 <= b2; ++a2) a[upk(a2, b2)] += xv[a2] * xv[b2];
+            }
+          }
+          const int nterm = (j < l) ? R : RR;
+#pragma unroll
+          for (int e = 0; e < RR; ++e) {
+            if (e >= nterm) break;
+            const double v = wave_sum(a[e]);
+            if (lane == 0) d.Ccorr[(base + (j < l ? j * R + e : l * R + e)) * G + g] = v;
+          }
+        }
+      }
     }
   }
 }
@@ -498,255 +501,230 @@ __device__ double explicit_rss_thread(const DevScan& d, int y, int s, const doub
 }
 
 // ---------------------------------------------------------------------------
-// The pair solve.  A wave covers 64 traits (one per lane) and SPT
-// consecutive SNPs: the SNP's Gram pieces are wave-uniform (scalar loads),
-// and each lane loads its trait's data -- zt, RSS0 and, for l <= 8, the
-// pattern's covariate factor U_A -- once for all SPT SNPs.
-constexpr int EPI_SPT = 4;
-
+// One thread per (trait, SNP) pair: a wave covers 64 traits of one SNP, so
+// the SNP's Gram pieces are wave-uniform.
 template <int R, int LB, typename CT>
 __global__ __launch_bounds__(256) void k_epilogue(DevScan d) {
-  constexpr int RR = R * (R + 1) / 2;
-  constexpr int P = LB + R;
-  constexpr bool UA_REG = LB <= 8;
-  constexpr int NUA = UA_REG ? LB * (LB + 1) / 2 : 1;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int s0w = (blockIdx.y * 4 + w) * EPI_SPT;
-  if (s0w >= d.mc) return;
+  const int s = blockIdx.y * 4 + w;
+  if (s >= d.mc) return;
   const int y = blockIdx.x * 64 + lane;
   if (y >= d.t) return;
+  constexpr int RR = R * (R + 1) / 2;
+  constexpr int P = LB + R;
   const int l = d.l, G = d.G;
   const int g = d.ypat[y];
   const int cnt = d.pat_cnt[g];
-  const bool pd = d.pat_pd[g] != 0;
-  const double rss0 = d.rss0[y], yyv = d.yy[y];
-  const bool yzero = d.yzero[y] > 0;
   const CT* C = reinterpret_cast<const CT*>(d.Cbuf);
 
-  double zt[LB];
+  double rb[R];
 #pragma unroll
-  for (int j = 0; j < LB; ++j) zt[j] = (j < l) ? d.zt[(size_t)j * d.ldt + y] : 0.0;
-  double ua[NUA], uinv[LB];
+  for (int c = 0; c < R; ++c) rb[c] = (double)C[(size_t)(s * R + c) * d.ldc + y];
+
+  // Off-diagonal block B_q as the reference assembles it: every column's
+  // correction is subtracted from the first SNP column
+  // (src/Algorithm.cpp:1390-1396).
+  double X[LB][R];
+  double Clr[R > 1 ? LB : 1][R];
+  double Crr[RR];
 #pragma unroll
   for (int j = 0; j < LB; ++j) {
-    uinv[j] = (j < l) ? d.UA_invd[(size_t)j * G + g] : 0.0;
-    if (UA_REG)
 #pragma unroll
-      for (int i = 0; i <= j; ++i) ua[UA_REG ? upk(i, j) : 0] = (j < l) ? d.UA[(size_t)upk(i, j) * G + g] : 0.0;
+    for (int c = 0; c < R; ++c) {
+      X[j][c] = (j < l) ? d.Str[((size_t)s * l + j) * R + c] : 0.0;
+      if (R > 1) Clr[R > 1 ? j : 0][c] = 0.0;
+    }
   }
-  auto UAv = [&](int i, int j) -> double {
-    if (UA_REG) return ua[UA_REG ? upk(i, j) : 0];
-    return d.UA[(size_t)upk(i, j) * G + g];
-  };
-
-  for (int q = 0; q < EPI_SPT; ++q) {
-    const int s = s0w + q;
-    if (s >= d.mc) break;
-    double rb[R];
 #pragma unroll
-    for (int c = 0; c < R; ++c) rb[c] = (double)C[(size_t)(s * R + c) * d.ldc + y];
-
-    // Off-diagonal block B_q as the reference assembles it: every column's
-    // correction is subtracted from the first SNP column
-    // (src/Algorithm.cpp:1390-1396).
-    double X[LB][R];
-    double Clr[R > 1 ? LB : 1][R];
-    double Crr[RR];
-#pragma unroll
-    for (int j = 0; j < LB; ++j) {
-#pragma unroll
-      for (int c = 0; c < R; ++c) {
-        X[j][c] = (j < l) ? d.Str[((size_t)s * l + j) * R + c] : 0.0;
-        if (R > 1) Clr[R > 1 ? j : 0][c] = 0.0;
-      }
-    }
-#pragma unroll
-    for (int e = 0; e < RR; ++e) Crr[e] = 0.0;
-    if (cnt > 0) {
-      const size_t base = (size_t)s * d.Q;
-#pragma unroll
-      for (int j = 0; j < LB; ++j) {
-        if (j < l) {
-          double tot = 0.0;
-#pragma unroll
-          for (int c = 0; c < R; ++c) {
-            const double v = d.Ccorr[(base + j * R + c) * G + g];
-            tot += v;
-            if (R > 1) Clr[R > 1 ? j : 0][c] = v;
-          }
-          X[j][0] -= tot;
-        }
-      }
-#pragma unroll
-      for (int e = 0; e < RR; ++e) Crr[e] = d.Ccorr[(base + l * R + e) * G + g];
-    }
-    // D_q: Sbr corrected only in rows j < min(l, r) (src/Algorithm.cpp:1300).
-    double Sc[RR];
-#pragma unroll
-    for (int b2 = 0; b2 < R; ++b2)
-#pragma unroll
-      for (int a = 0; a <= b2; ++a)
-        Sc[upk(a, b2)] = d.Sbr[(size_t)s * RR + upk(a, b2)] - ((a < l) ? Crr[upk(a, b2)] : 0.0);
-
-    // X = U_A^-T B_q
+  for (int e = 0; e < RR; ++e) Crr[e] = 0.0;
+  if (cnt > 0) {
+    const size_t base = (size_t)s * d.Q;
 #pragma unroll
     for (int j = 0; j < LB; ++j) {
       if (j < l) {
+        double tot = 0.0;
 #pragma unroll
         for (int c = 0; c < R; ++c) {
-          double v = X[j][c];
-#pragma unroll
-          for (int i = 0; i < j; ++i) v -= UAv(i, j) * X[i][c];
-          X[j][c] = v * uinv[j];
+          const double v = d.Ccorr[(base + j * R + c) * G + g];
+          tot += v;
+          if (R > 1) Clr[R > 1 ? j : 0][c] = v;
         }
+        X[j][0] -= tot;
       }
     }
-    // Schur complement and its Cholesky factor.
 #pragma unroll
-    for (int b2 = 0; b2 < R; ++b2)
+    for (int e = 0; e < RR; ++e) Crr[e] = d.Ccorr[(base + l * R + e) * G + g];
+  }
+  // D_q: Sbr corrected only in rows j < min(l, r) (src/Algorithm.cpp:1300).
+  double Sc[RR];
 #pragma unroll
-      for (int a = 0; a <= b2; ++a) {
-        double v = Sc[upk(a, b2)];
+  for (int b2 = 0; b2 < R; ++b2)
 #pragma unroll
-        for (int j = 0; j < LB; ++j) v -= X[j][a] * X[j][b2];
-        Sc[upk(a, b2)] = v;
-      }
-    bool ok = pd;
-#pragma unroll
-    for (int j = 0; j < R; ++j) {
-      double ajj = Sc[upk(j, j)];
-#pragma unroll
-      for (int k = 0; k < j; ++k) ajj -= Sc[upk(k, j)] * Sc[upk(k, j)];
-      if (!(ajj > 0.0)) ok = false;
-      ajj = sqrt(ajj);
-      Sc[upk(j, j)] = ajj;
-#pragma unroll
-      for (int i = j + 1; i < R; ++i) {
-        double aji = Sc[upk(j, i)];
-#pragma unroll
-        for (int k = 0; k < j; ++k) aji -= Sc[upk(k, j)] * Sc[upk(k, i)];
-        Sc[upk(j, i)] = aji / ajj;
-      }
-    }
+    for (int a = 0; a <= b2; ++a)
+      Sc[upk(a, b2)] = d.Sbr[(size_t)s * RR + upk(a, b2)] - ((a < l) ? Crr[upk(a, b2)] : 0.0);
 
-    double b[P];
+  // X = U_A^-T B_q
 #pragma unroll
-    for (int h = 0; h < P; ++h) b[h] = 0.0;
-    double rss = 0.0;
-    if (ok) {
-      double z[R];
+  for (int j = 0; j < LB; ++j) {
+    if (j < l) {
+      const double inv = d.UA_invd[(size_t)j * G + g];
 #pragma unroll
       for (int c = 0; c < R; ++c) {
-        double v = rb[c];
+        double v = X[j][c];
 #pragma unroll
-        for (int j = 0; j < LB; ++j) v -= X[j][c] * zt[j];
-        z[c] = v;
-      }
-#pragma unroll
-      for (int i = 0; i < R; ++i) {   // U_S' z = num
-        double v = z[i];
-#pragma unroll
-        for (int k = 0; k < i; ++k) v -= Sc[upk(k, i)] * z[k];
-        z[i] = v / Sc[upk(i, i)];
-      }
-      double zz = 0.0;
-#pragma unroll
-      for (int c = 0; c < R; ++c) zz += z[c] * z[c];
-      double br[R];
-#pragma unroll
-      for (int i = R - 1; i >= 0; --i) {   // U_S b_r = z
-        double v = z[i];
-#pragma unroll
-        for (int k = i + 1; k < R; ++k) v -= Sc[upk(i, k)] * br[k];
-        br[i] = v / Sc[upk(i, i)];
-      }
-      rss = rss0 - zz;
-#pragma unroll
-      for (int c = 0; c < R; ++c) b[LB + c] = br[c];
-      const bool quirk = (R > 1) && cnt > 0;
-      const bool need_bl = d.need_bl || quirk || yzero || !(rss >= RSS_GUARD * yyv);
-      if (need_bl) {
-        // b_l = U_A^-1 (zt - X b_r)
-#pragma unroll
-        for (int j = LB - 1; j >= 0; --j) {
-          if (j < l) {
-            double v = zt[j];
-#pragma unroll
-            for (int c = 0; c < R; ++c) v -= X[j][c] * br[c];
-#pragma unroll
-            for (int i = j + 1; i < LB; ++i)
-              if (i < l) v -= UAv(j, i) * b[i];
-            b[j] = v * uinv[j];
-          }
-        }
-      }
-      if (quirk) {
-        // RSS of the reference's b on the true masked Gram matrix: add
-        // b' (S_true - S_q) b, nonzero in the B block (first SNP column got
-        // every correction) and in D rows >= l (never corrected).
-        double qv = 0.0;
-#pragma unroll
-        for (int j = 0; j < LB; ++j) {
-          if (j < l) {
-            double rest = 0.0;
-#pragma unroll
-            for (int c = 1; c < R; ++c) rest += Clr[R > 1 ? j : 0][c];
-            double v = rest * br[0];
-#pragma unroll
-            for (int c = 1; c < R; ++c) v -= Clr[R > 1 ? j : 0][c] * br[c];
-            qv += 2.0 * b[j] * v;
-          }
-        }
-#pragma unroll
-        for (int b2 = 0; b2 < R; ++b2)
-#pragma unroll
-          for (int a = 0; a <= b2; ++a)
-            if (a >= l) qv -= (a == b2 ? 1.0 : 2.0) * br[a] * Crr[upk(a, b2)] * br[b2];
-        rss += qv;
+        for (int i = 0; i < j; ++i) v -= d.UA[(size_t)upk(i, j) * G + g] * X[i][c];
+        X[j][c] = v * inv;
       }
     }
-    // Compact coefficient vector: b[0..l) covariates, b[l..l+R) SNP terms.
-    double bc[P];
+  }
+  // Schur complement and its Cholesky factor.
 #pragma unroll
-    for (int h = 0; h < P; ++h) bc[h] = 0.0;
+  for (int b2 = 0; b2 < R; ++b2)
 #pragma unroll
-    for (int h = 0; h < LB; ++h)
-      if (h < l) bc[h] = b[h];
+    for (int a = 0; a <= b2; ++a) {
+      double v = Sc[upk(a, b2)];
+#pragma unroll
+      for (int j = 0; j < LB; ++j) v -= X[j][a] * X[j][b2];
+      Sc[upk(a, b2)] = v;
+    }
+  bool ok = d.pat_pd[g] != 0;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    double ajj = Sc[upk(j, j)];
+#pragma unroll
+    for (int k = 0; k < j; ++k) ajj -= Sc[upk(k, j)] * Sc[upk(k, j)];
+    if (!(ajj > 0.0)) ok = false;
+    ajj = sqrt(ajj);
+    Sc[upk(j, j)] = ajj;
+#pragma unroll
+    for (int i = j + 1; i < R; ++i) {
+      double aji = Sc[upk(j, i)];
+#pragma unroll
+      for (int k = 0; k < j; ++k) aji -= Sc[upk(k, j)] * Sc[upk(k, i)];
+      Sc[upk(j, i)] = aji / ajj;
+    }
+  }
+  double zt[LB];
+#pragma unroll
+  for (int j = 0; j < LB; ++j) zt[j] = (j < l) ? d.zt[(size_t)j * d.ldt + y] : 0.0;
+
+  double b[P];
+#pragma unroll
+  for (int h = 0; h < P; ++h) b[h] = 0.0;
+  double rss = 0.0;
+  if (ok) {
+    double z[R];
 #pragma unroll
     for (int c = 0; c < R; ++c) {
+      double v = rb[c];
 #pragma unroll
-      for (int h = 0; h < P; ++h)
-        if (h == l + c) bc[h] = b[LB + c];
+      for (int j = 0; j < LB; ++j) v -= X[j][c] * zt[j];
+      z[c] = v;
     }
-    const bool fallback = !ok || yzero || !(rss >= RSS_GUARD * yyv);
-    if (fallback) {
-      if (!ok) {   // sposv left the right-hand side in place
 #pragma unroll
-        for (int h = 0; h < P; ++h) {
-          if (h < l) bc[h] = d.rt[(size_t)h * d.ldt + y];
+    for (int i = 0; i < R; ++i) {   // U_S' z = num
+      double v = z[i];
+#pragma unroll
+      for (int k = 0; k < i; ++k) v -= Sc[upk(k, i)] * z[k];
+      z[i] = v / Sc[upk(i, i)];
+    }
+    double zz = 0.0;
+#pragma unroll
+    for (int c = 0; c < R; ++c) zz += z[c] * z[c];
+    double br[R];
+#pragma unroll
+    for (int i = R - 1; i >= 0; --i) {   // U_S b_r = z
+      double v = z[i];
+#pragma unroll
+      for (int k = i + 1; k < R; ++k) v -= Sc[upk(i, k)] * br[k];
+      br[i] = v / Sc[upk(i, i)];
+    }
+    rss = d.rss0[y] - zz;
+#pragma unroll
+    for (int c = 0; c < R; ++c) b[LB + c] = br[c];
+    const bool quirk = (R > 1) && cnt > 0;
+    const bool need_bl = d.need_bl || quirk || d.yzero[y] > 0 || !(rss >= RSS_GUARD * d.yy[y]);
+    if (need_bl) {
+      // b_l = U_A^-1 (zt - X b_r)
+#pragma unroll
+      for (int j = LB - 1; j >= 0; --j) {
+        if (j < l) {
+          double v = zt[j];
+#pragma unroll
+          for (int c = 0; c < R; ++c) v -= X[j][c] * br[c];
+#pragma unroll
+          for (int i = j + 1; i < LB; ++i)
+            if (i < l) v -= d.UA[(size_t)upk(j, i) * G + g] * b[i];
+          b[j] = v * d.UA_invd[(size_t)j * G + g];
         }
-#pragma unroll
-        for (int c = 0; c < R; ++c)
-#pragma unroll
-          for (int h = 0; h < P; ++h)
-            if (h == l + c) bc[h] = rb[c];
       }
-      const unsigned long long slot = atomicAdd(&d.counters[CNT_FB_CHUNK], 1ull);
-      atomicAdd(&d.counters[CNT_FALLBACK], 1ull);
-      if (slot < (unsigned long long)d.fb_cap) {
-        FallbackEntry* f = d.fb + slot;
-        f->y = y;
-        f->s = s;
-        f->flags = ok ? 0 : 1;
-#pragma unroll
-        for (int h = 0; h < P; ++h) f->b[h] = bc[h];
-        continue;
-      }
-      rss = explicit_rss_thread<R, LB>(d, y, s, bc);
     }
-    stats_emit<P>(d, y, s, bc, rss);
+    if (quirk) {
+      // RSS of the reference's b on the true masked Gram matrix: add
+      // b' (S_true - S_q) b, nonzero in the B block (first SNP column got
+      // every correction) and in D rows >= l (never corrected).
+      double q = 0.0;
+#pragma unroll
+      for (int j = 0; j < LB; ++j) {
+        if (j < l) {
+          double rest = 0.0;
+#pragma unroll
+          for (int c = 1; c < R; ++c) rest += Clr[R > 1 ? j : 0][c];
+          double v = rest * br[0];
+#pragma unroll
+          for (int c = 1; c < R; ++c) v -= Clr[R > 1 ? j : 0][c] * br[c];
+          q += 2.0 * b[j] * v;
+        }
+      }
+#pragma unroll
+      for (int b2 = 0; b2 < R; ++b2)
+#pragma unroll
+        for (int a = 0; a <= b2; ++a)
+          if (a >= l) q -= (a == b2 ? 1.0 : 2.0) * br[a] * Crr[upk(a, b2)] * br[b2];
+      rss += q;
+    }
   }
+  // Compact coefficient vector: b[0..l) covariates, b[l..l+R) SNP terms.
+  double bc[P];
+#pragma unroll
+  for (int h = 0; h < P; ++h) bc[h] = 0.0;
+#pragma unroll
+  for (int h = 0; h < LB; ++h)
+    if (h < l) bc[h] = b[h];
+#pragma unroll
+  for (int c = 0; c < R; ++c) {
+#pragma unroll
+    for (int h = 0; h < P; ++h)
+      if (h == l + c) bc[h] = b[LB + c];
+  }
+  const bool fallback = !ok || d.yzero[y] > 0 || !(rss >= RSS_GUARD * d.yy[y]);
+  if (fallback) {
+    if (!ok) {   // sposv left the right-hand side in place
+#pragma unroll
+      for (int h = 0; h < P; ++h) {
+        if (h < l) bc[h] = d.rt[(size_t)h * d.ldt + y];
+      }
+#pragma unroll
+      for (int c = 0; c < R; ++c)
+#pragma unroll
+        for (int h = 0; h < P; ++h)
+          if (h == l + c) bc[h] = rb[c];
+    }
+    const unsigned long long slot = atomicAdd(&d.counters[CNT_FB_CHUNK], 1ull);
+    atomicAdd(&d.counters[CNT_FALLBACK], 1ull);
+    if (slot < (unsigned long long)d.fb_cap) {
+      FallbackEntry* f = d.fb + slot;
+      f->y = y;
+      f->s = s;
+      f->flags = ok ? 0 : 1;
+#pragma unroll
+      for (int h = 0; h < P; ++h) f->b[h] = bc[h];
+      return;
+    }
+    rss = explicit_rss_thread<R, LB>(d, y, s, bc);
+  }
+  stats_emit<P>(d, y, s, bc, rss);
 }
 
 // Explicit residual pass for the pairs the epilogue handed over: one wave
@@ -797,43 +775,25 @@ __global__ void k_reset_counter(unsigned long long* c, int idx) { c[idx] = 0ull;
 
 // ---------------------------------------------------------------------------
 // launchers
-size_t prep_corr_lds_bytes(int G, int Q) { return sizeof(double) * (size_t)G * PREP_SPW * Q; }
-
 template <int R>
-static void prep_r(const DevScan& d, int prec, void* Ahi, void* Alo, const unsigned long long* rowmask,
-                   hipStream_t st) {
+static void prep_r(const DevScan& d, int prec, void* Ahi, void* Alo, bool corr, hipStream_t st) {
   dim3 grid((d.mc + PREP_SPW - 1) / PREP_SPW), block(256);
-  const bool corr = rowmask != nullptr;
-  const size_t sh = corr ? prep_corr_lds_bytes(d.G, d.Q) : 0;
-#define OA_PR(OP_)                                                                                   \
-  if (corr) hipLaunchKernelGGL((k_prep<R, OP_, true>), grid, block, sh, st, d, Ahi, Alo, rowmask);   \
-  else hipLaunchKernelGGL((k_prep<R, OP_, false>), grid, block, 0, st, d, Ahi, Alo, rowmask);
+#define OA_PR(OP_)                                                                      \
+  if (corr) hipLaunchKernelGGL((k_prep<R, OP_, true>), grid, block, 0, st, d, Ahi, Alo); \
+  else hipLaunchKernelGGL((k_prep<R, OP_, false>), grid, block, 0, st, d, Ahi, Alo);
   if (prec == PREC_FP64) { OA_PR(PREC_FP64) }
   else if (prec == PREC_FP32) { OA_PR(PREC_FP32) }
   else { OA_PR(PREC_BF16X3) }
 #undef OA_PR
 }
 
-void launch_prep(const DevScan& d, int prec, void* Ahi, void* Alo, const unsigned long long* rowmask,
-                 hipStream_t st) {
+void launch_prep(const DevScan& d, int prec, void* Ahi, void* Alo, bool corr, hipStream_t st) {
   switch (d.r) {
-    case 1: prep_r<1>(d, prec, Ahi, Alo, rowmask, st); break;
-    case 2: prep_r<2>(d, prec, Ahi, Alo, rowmask, st); break;
-    case 3: prep_r<3>(d, prec, Ahi, Alo, rowmask, st); break;
-    default: prep_r<4>(d, prec, Ahi, Alo, rowmask, st); break;
+    case 1: prep_r<1>(d, prec, Ahi, Alo, corr, st); break;
+    case 2: prep_r<2>(d, prec, Ahi, Alo, corr, st); break;
+    case 3: prep_r<3>(d, prec, Ahi, Alo, corr, st); break;
+    default: prep_r<4>(d, prec, Ahi, Alo, corr, st); break;
   }
-}
-
-__global__ void k_rowmask(int G, int n, const int* pat_cnt, const int* pat_rep, const int* rows,
-                          unsigned long long* mask) {
-  const int g = blockIdx.x;
-  const int* r = rows + (size_t)pat_rep[g] * n;
-  for (int q = threadIdx.x; q < pat_cnt[g]; q += blockDim.x) atomicOr(&mask[r[q]], 1ull << g);
-}
-
-void launch_rowmask(int G, int n, const int* pat_cnt, const int* pat_rep, const int* rows,
-                    unsigned long long* mask, hipStream_t st) {
-  hipLaunchKernelGGL(k_rowmask, dim3(G), dim3(256), 0, st, G, n, pat_cnt, pat_rep, rows, mask);
 }
 
 // (R, LB) pairs with compiled kernels: R=1: LB 8/16/32; R=2: 8/16; R=3,4: 8.
@@ -863,7 +823,7 @@ void launch_mask(const DevScan& d, int lb, hipStream_t st) {
 }
 
 void launch_epilogue(const DevScan& d, int prec, int lb, hipStream_t st) {
-  dim3 grid((d.t + 63) / 64, (d.mc + 4 * EPI_SPT - 1) / (4 * EPI_SPT)), block(256);
+  dim3 grid((d.t + 63) / 64, (d.mc + 3) / 4), block(256);
 #define OA_EP(R_, LB_)                                                                          \
   if (prec == PREC_FP64) hipLaunchKernelGGL((k_epilogue<R_, LB_, double>), grid, block, 0, st, d); \
   else hipLaunchKernelGGL((k_epilogue<R_, LB_, float>), grid, block, 0, st, d);

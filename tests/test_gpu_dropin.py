@@ -144,3 +144,39 @@ def test_dropin_files_interactions(oracle, fx, tmp_path, multi):
     assert (s.r, s.m) == (d["r"], d["m"])
     rows = read_dis(os.path.join(tmp, "inter_dis.txt"))
     assert len(rows) > 0
+
+
+def test_dropin_streams_several_genotype_blocks(oracle, tmp_path, monkeypatch):
+    """1200 SNPs in genotype blocks of 500 (three uploads, the next one
+    queued while the current one is scanned): the outputs equal a single
+    oracle pass, in the reference's (trait block, SNP block) order."""
+    from omicabelnomm_amd import Algorithm, Outputs, Settings, synth
+    tmp = str(tmp_path)
+    data = synth.make(n=300, l=3, m=1200, t=12, miss="mcar", miss_rate=0.05, planted=20, beta=0.5, seed=17)
+    XL, XR, Y = data.XL, data.xr_matrix(), data.y_matrix()
+    obs = ["id%d" % i for i in range(data.n)]
+    databel.write(os.path.join(tmp, "XL"), XL.astype(np.float32), obs, ["c%d" % i for i in range(3)])
+    databel.write(os.path.join(tmp, "XR"), XR.astype(np.float32), obs, ["snp%d" % i for i in range(1200)])
+    databel.write(os.path.join(tmp, "Y"), Y.astype(np.float32), obs, ["y%d" % i for i in range(12)])
+    monkeypatch.setenv("OMICABELNOMM_BLOCK_SNPS", "500")
+    s = Settings()
+    Algorithm().applyDefaultParams(s)
+    s.fnameAL, s.fnameAR, s.fnameY = (os.path.join(tmp, x) for x in ("XL", "XR", "Y"))
+    s.fnameOutFiles = os.path.join(tmp, "blk")
+    s.minPstore, s.minPdisp, s.minR2store, s.minR2disp = 0.01, 0.01, -10.0, -10.0
+    s.storeBin = True
+    out = Outputs()
+    Algorithm().solve(s, out)
+    f32 = lambda a: a.astype(np.float32).astype(np.float64)
+    o = oracle.scan(f32(XL), f32(XR), f32(Y), 1, min_p_store=0.01, min_p_disp=0.01, min_r2_store=-10.0,
+                    threads=4)
+    assert out.total_sig_results == o["n_sig"]
+    rows = read_dis(os.path.join(tmp, "blk_dis.txt"))
+    names = dict(y=["y%d" % i for i in range(12)], al=["c%d" % i for i in range(3)],
+                 ar=["snp%d" % i for i in range(1200)])
+    want = expected_dis_rows(o, names, 3, 1, 0.01, -10.0, data.n)
+    assert [(r["phe"], r["snp"]) for r in rows] == [(w["phe"], w["snp"]) for w in want]
+    recs = read_dbin(os.path.join(tmp, "blk_sto.dbin"), disp_cov=False, store_all=False, l=3, r=1)
+    want_recs = expected_records(o, l=3, r=1)
+    assert [(r["y"], r["ar_offset"], len(r["results"])) for r in recs] == \
+        [(w[0], w[1], len(w[3])) for w in want_recs]
