@@ -167,7 +167,8 @@ int oagpu_host_alloc(size_t bytes, void **ptr);
 void oagpu_host_free(void *ptr);
 
 /* Device-side timing of the last oagpu_scan, per kernel family (ms):
- * out[0] prep, [1] gemm, [2] snp gram, [3] corrections, [4] epilogue,
+ * out[0] prep, [1] gemm, [2] snp gram, [3] corrections and per-(SNP,
+ * pattern) solve pieces, [4] epilogue,
  * [5] fallback, [6] whole scan.  Returns the number of values written. */
 int oagpu_last_timings(const oagpu_ctx *ctx, double *out, int n);
 

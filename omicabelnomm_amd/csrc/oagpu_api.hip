@@ -31,6 +31,7 @@ struct oagpu_ctx {
   bool have_cov = false;
   std::vector<double> Stl_inv_diag;
   double *XL_cm = nullptr, *XL_rm = nullptr, *U0 = nullptr, *Stl_inv_d = nullptr, *Stl_pk = nullptr;
+  std::vector<double> XL_rm_h;   // [n][l]
   void *XLop_hi = nullptr, *XLop_lo = nullptr;   // covariate rows of B, in the operand format
 
   // interactions
@@ -49,11 +50,18 @@ struct oagpu_ctx {
   void *Bhi = nullptr, *Blo = nullptr;
   int* rows = nullptr;
   int *ymiss = nullptr, *yzero = nullptr, *ypat = nullptr;
-  double *yy = nullptr, *sst = nullptr, *rss0 = nullptr, *rt = nullptr, *zt = nullptr;
+  double *yy = nullptr, *sst = nullptr, *rss0 = nullptr, *rt = nullptr, *zt = nullptr, *ay = nullptr;
   unsigned long long* hash = nullptr;
   int* scratch_i = nullptr;   // rep_of / mismatch
   int G = 0, max_pat_cnt = 0;
-  bool fused_corr = false;   // corrections computed in the operand pass
+  // Few missing rows in total: the operand pass copies the genotype values
+  // at U (the union of the patterns' missing rows) into Xu and k_corr reads
+  // them there; otherwise k_mask gathers them from the genotypes.
+  bool compact_corr = false;
+  int nU = 0;
+  int *uidx = nullptr, *pu_off = nullptr, *pu_list = nullptr;
+  double* XLu = nullptr;
+  size_t uidx_cap = 0, pu_off_cap = 0, pu_list_cap = 0, XLu_cap = 0;
   int *pat_cnt = nullptr, *pat_rep = nullptr, *pat_pd = nullptr;
   double *UA = nullptr, *UA_invd = nullptr;
   size_t pat_cap = 0;
@@ -93,7 +101,12 @@ struct oagpu_ctx {
   int mc_cap = 0;
   void *Ahi = nullptr, *Alo = nullptr, *Cbuf = nullptr;
   double *col_mean = nullptr, *Sbr = nullptr, *Str = nullptr, *varx = nullptr, *Ccorr = nullptr;
-  size_t Ahi_cap = 0, Alo_cap = 0, C_cap = 0, Ccorr_cap = 0;
+  double* Xu = nullptr;
+  long long ldxu = 0;
+  // few patterns per trait block: per-(SNP, pattern) solve pieces
+  bool use_pst = false;
+  double* pst = nullptr;
+  size_t Ahi_cap = 0, Alo_cap = 0, C_cap = 0, Ccorr_cap = 0, Xu_cap = 0, pst_cap = 0;
 
   // outputs
   oagpu_hit* hits = nullptr;
@@ -236,9 +249,12 @@ static DevScan make_dev(const oagpu_ctx* c) {
   d.E = c->E; d.lde = c->lde;
   d.Y = c->Y; d.ydt = c->ydt; d.ldy = c->ldy;
   d.ymiss = c->ymiss; d.yzero = c->yzero; d.yy = c->yy; d.sst = c->sst; d.rss0 = c->rss0;
-  d.ypat = c->ypat; d.rt = c->rt; d.zt = c->zt;
+  d.ypat = c->ypat; d.rt = c->rt; d.zt = c->zt; d.ay = c->ay;
   d.pat_cnt = c->pat_cnt; d.pat_rep = c->pat_rep; d.rows = c->rows;
   d.UA = c->UA; d.UA_invd = c->UA_invd; d.pat_pd = c->pat_pd;
+  d.nU = c->compact_corr ? c->nU : 0;
+  d.uidx = c->uidx; d.XLu = c->XLu; d.pu_off = c->pu_off; d.pu_list = c->pu_list;
+  d.Xu = c->Xu; d.ldxu = c->ldxu; d.pst = c->pst;
   d.Gm = c->Gm; d.gdt = c->gdt; d.ldg = c->ldg; d.col_src = c->col_src; d.col_env = c->col_env;
   d.col_mean = c->col_mean; d.Sbr = c->Sbr; d.Str = c->Str; d.varx = c->varx; d.Ccorr = c->Ccorr;
   d.Cbuf = c->Cbuf;
@@ -307,6 +323,7 @@ extern "C" int oagpu_create(const oagpu_params* prm, oagpu_ctx** out) {
   HIPCK(c, hipMalloc(&c->rss0, sizeof(double) * T));
   HIPCK(c, hipMalloc(&c->rt, sizeof(double) * (size_t)l * c->ldt));
   HIPCK(c, hipMalloc(&c->zt, sizeof(double) * (size_t)l * c->ldt));
+  HIPCK(c, hipMalloc(&c->ay, sizeof(double) * (size_t)l * c->ldt));
   HIPCK(c, hipMalloc(&c->hash, sizeof(unsigned long long) * 2 * T));
   return OAGPU_OK;
 }
@@ -330,9 +347,10 @@ extern "C" void oagpu_destroy(oagpu_ctx* c) {
   for (void* b : dense_bufs) dfree(b);
   void* bufs[] = {c->XL_cm, c->XL_rm, c->U0, c->Stl_inv_d, c->Stl_pk, c->XLop_hi, c->XLop_lo, c->E,
                   c->Y_own, c->Bhi, c->Blo, c->rows, c->ymiss, c->yzero, c->ypat, c->yy, c->sst,
-                  c->rss0, c->rt, c->zt, c->hash, c->scratch_i, c->pat_cnt, c->pat_rep, c->pat_pd,
+                  c->rss0, c->rt, c->zt, c->ay, c->hash, c->scratch_i, c->pat_cnt, c->pat_rep, c->pat_pd,
                   c->UA, c->UA_invd, c->Ahi, c->Alo, c->Cbuf,
-                  c->col_mean, c->Sbr, c->Str, c->varx, c->Ccorr, c->hits, c->counters, c->fb};
+                  c->col_mean, c->Sbr, c->Str, c->varx, c->Ccorr, c->hits, c->counters, c->fb,
+                  c->uidx, c->pu_off, c->pu_list, c->XLu, c->Xu, c->pst};
   for (void* b : bufs) dfree(b);
   if (c->st) (void)hipStreamDestroy(c->st);
   if (c->cst) (void)hipStreamDestroy(c->cst);
@@ -343,6 +361,7 @@ extern "C" const char* oagpu_last_error(const oagpu_ctx* c) { return c ? c->err.
 
 extern "C" int oagpu_set_covariates(oagpu_ctx* c, const double* XL, long long ld) {
   if (!c || !XL || ld < c->n) return c ? fail(c, OAGPU_ERR_ARG, "bad covariates") : OAGPU_ERR_ARG;
+  if (c->pending) return fail(c, OAGPU_ERR_STATE, "a scan is in flight (oagpu_scan_wait first)");
   HIPCK(c, hipSetDevice(c->prm.device));
   const int n = c->n, l = c->l, np = c->n_pad;
   std::vector<double> S(l * l);
@@ -398,6 +417,7 @@ extern "C" int oagpu_set_covariates(oagpu_ctx* c, const double* XL, long long ld
   }
   HIPCK(c, hipMemcpy(c->XL_cm, cm.data(), sizeof(double) * cm.size(), hipMemcpyHostToDevice));
   HIPCK(c, hipMemcpy(c->XL_rm, rm.data(), sizeof(double) * rm.size(), hipMemcpyHostToDevice));
+  c->XL_rm_h = rm;
   HIPCK(c, hipMemcpy(c->U0, U.data(), sizeof(double) * U.size(), hipMemcpyHostToDevice));
   HIPCK(c, hipMemcpy(c->Stl_pk, pk.data(), sizeof(double) * pk.size(), hipMemcpyHostToDevice));
   HIPCK(c, hipMemcpy(c->Stl_inv_d, c->Stl_inv_diag.data(), sizeof(double) * l, hipMemcpyHostToDevice));
@@ -434,6 +454,7 @@ extern "C" int oagpu_set_covariates(oagpu_ctx* c, const double* XL, long long ld
 
 extern "C" int oagpu_set_interactions(oagpu_ctx* c, const double* E, long long ld, int n_env) {
   if (!c || !E || ld < c->n || n_env <= 0) return c ? fail(c, OAGPU_ERR_ARG, "bad interactions") : OAGPU_ERR_ARG;
+  if (c->pending) return fail(c, OAGPU_ERR_STATE, "a scan is in flight (oagpu_scan_wait first)");
   HIPCK(c, hipSetDevice(c->prm.device));
   dfree(c->E);
   c->E = nullptr;
@@ -453,10 +474,13 @@ extern "C" int oagpu_set_interactions(oagpu_ctx* c, const double* E, long long l
 
 static size_t dt_size(int dt) { return dt == OAGPU_F64 ? 8 : (dt == OAGPU_F32 ? 4 : 2); }
 
+static int build_union(oagpu_ctx* c, const std::vector<int>& pat_rep, const std::vector<int>& pat_cnt);
+
 extern "C" int oagpu_load_traits(oagpu_ctx* c, const void* Y, int dtype, int where, long long ld,
                                  int t, int trait_offset) {
   if (!c) return OAGPU_ERR_ARG;
   if (!c->have_cov) return fail(c, OAGPU_ERR_STATE, "oagpu_set_covariates must come first");
+  if (c->pending) return fail(c, OAGPU_ERR_STATE, "a scan is in flight (oagpu_scan_wait first)");
   if (!Y || t <= 0 || t > c->prm.max_traits || ld < c->n || dtype < 0 || dtype > 2 || dtype == OAGPU_F16)
     return fail(c, OAGPU_ERR_ARG, "bad trait block (t <= max_traits, ld >= n, dtype F64/F32)");
   HIPCK(c, hipSetDevice(c->prm.device));
@@ -540,19 +564,60 @@ extern "C" int oagpu_load_traits(oagpu_ctx* c, const void* Y, int dtype, int whe
   d = make_dev(c);
   launch_pattern(d, c->Stl_pk, G, c->pat_rep, c->UA, c->UA_invd, c->pat_pd, c->st);
   HIPCK(c, hipGetLastError());
-  // Few missing rows in total: the corrections are computed in the operand
-  // pass; many patterns (e.g. independent missingness per trait) use the
-  // separate SNP-tiled kernel.
   {
-    long long tot = 0;
-    for (int g = 0; g < G; ++g) tot += pat_cnt[g];
-    c->fused_corr = c->max_pat_cnt > 0 && tot <= 32768;
+    int rc = build_union(c, pat_rep, pat_cnt);
+    if (rc) return rc;
   }
   d = make_dev(c);
-  launch_trait_solve(d, c->lb, c->zt, c->rss0, c->st);
+  launch_trait_solve(d, c->lb, c->zt, c->ay, c->rss0, c->st);
   HIPCK(c, hipGetLastError());
   HIPCK(c, hipStreamSynchronize(c->st));
   c->have_traits = true;
+  return OAGPU_OK;
+}
+
+// U = union of the patterns' missing rows.  When it is small (at most a
+// quarter of the rows) the corrections use the compact path: uidx, the
+// covariate rows of U and each pattern's rows as positions in U.
+static int build_union(oagpu_ctx* c, const std::vector<int>& pat_rep, const std::vector<int>& pat_cnt) {
+  const int n = c->n, l = c->l, G = (int)pat_rep.size();
+  c->compact_corr = false;
+  c->nU = 0;
+  long long tot = 0;
+  for (int g = 0; g < G; ++g) tot += pat_cnt[g];
+  if (c->max_pat_cnt == 0 || tot > 8ll * n) return OAGPU_OK;
+  std::vector<int> rows_h((size_t)tot);
+  std::vector<int> off(G + 1, 0);
+  for (int g = 0; g < G; ++g) off[g + 1] = off[g] + pat_cnt[g];
+  for (int g = 0; g < G; ++g)
+    if (pat_cnt[g] > 0)
+      HIPCK(c, hipMemcpyAsync(rows_h.data() + off[g], c->rows + (size_t)pat_rep[g] * n, sizeof(int) * pat_cnt[g],
+                              hipMemcpyDeviceToHost, c->st));
+  HIPCK(c, hipStreamSynchronize(c->st));
+  std::vector<int> uidx(c->n_pad, -1);
+  for (int k : rows_h) {
+    if (k < 0 || k >= n) return fail(c, OAGPU_ERR_STATE, "missing-row list out of range");
+    uidx[k] = 0;
+  }
+  int nU = 0;
+  for (int k = 0; k < n; ++k)
+    if (uidx[k] == 0) uidx[k] = nU++;
+  if (nU > n / 4) return OAGPU_OK;
+  std::vector<double> XLu((size_t)nU * l);
+  for (int k = 0; k < n; ++k)
+    if (uidx[k] >= 0)
+      for (int j = 0; j < l; ++j) XLu[(size_t)uidx[k] * l + j] = c->XL_rm_h[(size_t)k * l + j];
+  for (auto& k : rows_h) k = uidx[k];
+  HIPCK(c, grow((void**)&c->uidx, &c->uidx_cap, sizeof(int) * uidx.size()));
+  HIPCK(c, grow((void**)&c->pu_off, &c->pu_off_cap, sizeof(int) * off.size()));
+  HIPCK(c, grow((void**)&c->pu_list, &c->pu_list_cap, sizeof(int) * std::max<size_t>(rows_h.size(), 1)));
+  HIPCK(c, grow((void**)&c->XLu, &c->XLu_cap, sizeof(double) * std::max<size_t>(XLu.size(), 1)));
+  HIPCK(c, hipMemcpy(c->uidx, uidx.data(), sizeof(int) * uidx.size(), hipMemcpyHostToDevice));
+  HIPCK(c, hipMemcpy(c->pu_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
+  HIPCK(c, hipMemcpy(c->pu_list, rows_h.data(), sizeof(int) * rows_h.size(), hipMemcpyHostToDevice));
+  HIPCK(c, hipMemcpy(c->XLu, XLu.data(), sizeof(double) * XLu.size(), hipMemcpyHostToDevice));
+  c->nU = nU;
+  c->compact_corr = true;
   return OAGPU_OK;
 }
 
@@ -640,6 +705,8 @@ static int ensure_chunk(oagpu_ctx* c, int mc) {
   const size_t abytes = arows * c->n_pad * c->op_size;
   const size_t cbytes = (size_t)mc * r * (c->t + l) * c->c_size;
   const size_t qbytes = sizeof(double) * (size_t)mc * Q * std::max(G, 1);
+  c->ldxu = ((long long)mc * r + 7) / 8 * 8;
+  const size_t xubytes = c->compact_corr ? sizeof(double) * (size_t)c->nU * c->ldxu : 0;
   if (mc > c->mc_cap) {
     dfree(c->col_mean); dfree(c->Sbr); dfree(c->Str); dfree(c->varx);
     HIPCK(c, hipMalloc(&c->col_mean, sizeof(double) * (size_t)mc * r));
@@ -652,6 +719,10 @@ static int ensure_chunk(oagpu_ctx* c, int mc) {
   if (c->prm.precision == OAGPU_BF16X3) HIPCK(c, grow(&c->Alo, &c->Alo_cap, abytes));
   HIPCK(c, grow(&c->Cbuf, &c->C_cap, cbytes));
   HIPCK(c, grow((void**)&c->Ccorr, &c->Ccorr_cap, qbytes));
+  if (xubytes) HIPCK(c, grow((void**)&c->Xu, &c->Xu_cap, xubytes));
+  if (c->use_pst)
+    HIPCK(c, grow((void**)&c->pst, &c->pst_cap,
+                  sizeof(double) * (size_t)mc * G * patsnp_width(r, c->lb)));
   return OAGPU_OK;
 }
 
@@ -660,10 +731,15 @@ static int scan_enqueue(oagpu_ctx* c, bool want_dense) {
   if (!c->have_traits || !c->have_geno) return fail(c, OAGPU_ERR_STATE, "load traits and genotypes before scanning");
   HIPCK(c, hipSetDevice(c->prm.device));
   const int r = c->r, p = c->p, t = c->t, m = c->m, l = c->l;
+  // The per-(SNP, pattern) pieces pay off when patterns are shared by
+  // several traits.
+  c->use_pst = (long long)c->G * 4 <= t;
   // Chunk size: bounded by max_snps and by ~12 GiB of chunk buffers.
   const size_t per_snp = (size_t)r * c->n_pad * c->op_size * (c->prm.precision == OAGPU_BF16X3 ? 2 : 1) +
                          (size_t)r * (t + l) * c->c_size + sizeof(double) * (size_t)c->Q * std::max(c->G, 1) +
-                         sizeof(double) * (size_t)(r + r * (r + 1) / 2 + l * r + p);
+                         sizeof(double) * (size_t)(r + r * (r + 1) / 2 + l * r + p) +
+                         (c->compact_corr ? sizeof(double) * (size_t)c->nU * r : 0) +
+                         (c->use_pst ? sizeof(double) * (size_t)c->G * patsnp_width(r, c->lb) : 0);
   long long mc = c->prm.max_snps > 0 ? c->prm.max_snps : (long long)((12ull << 30) / per_snp);
   mc = std::max(1ll, std::min(mc, (long long)m));
   mc = std::min(mc, (long long)(0x7fffffff / std::max(1, r * (t + l))));
@@ -703,17 +779,20 @@ static int scan_enqueue(oagpu_ctx* c, bool want_dense) {
     d.mc = mcc;
     hipEvent_t* E = &c->pev[(size_t)ci * 7];
     HIPCK(c, hipEventRecord(E[0], c->st));
-    launch_prep(d, c->prm.precision, c->Ahi, c->Alo, c->fused_corr, c->st);
+    launch_prep(d, c->prm.precision, c->Ahi, c->Alo, c->compact_corr, c->st);
     HIPCK(c, hipEventRecord(E[1], c->st));
     launch_gemm(c->prm.precision, c->Ahi, c->Alo, c->Bhi, c->Blo, c->Cbuf, mcc * r, t + l, c->n_pad, t + l,
                 c->st);
     HIPCK(c, hipEventRecord(E[2], c->st));
     launch_snp_gram(d, c->prm.precision, c->lb, c->st);
     HIPCK(c, hipEventRecord(E[3], c->st));
-    if (c->max_pat_cnt > 0 && !c->fused_corr) launch_mask(d, c->lb, c->st);
+    if (c->compact_corr) launch_corr(d, c->lb, c->st);
+    else if (c->max_pat_cnt > 0) launch_mask(d, c->lb, c->st);
+    if (c->use_pst) launch_patsnp(d, c->lb, c->st);
     HIPCK(c, hipEventRecord(E[4], c->st));
     launch_reset_counter(c->counters, CNT_FB_CHUNK, c->st);
-    launch_epilogue(d, c->prm.precision, c->lb, c->st);
+    if (c->use_pst) launch_epilogue_pst(d, c->prm.precision, c->lb, c->st);
+    else launch_epilogue(d, c->prm.precision, c->lb, c->st);
     HIPCK(c, hipEventRecord(E[5], c->st));
     launch_fallback(d, c->lb, c->st);
     HIPCK(c, hipEventRecord(E[6], c->st));

@@ -55,6 +55,7 @@ struct DevScan {
   const int* ypat;       // [t] pattern id
   const double* rt;      // [l][ldt] XL'y0
   const double* zt;      // [l][ldt] U_A^-T XL'y0
+  const double* ay;      // [l][ldt] U_A^-1 zt
 
   // patterns
   const int* pat_cnt;    // [G] missing rows
@@ -63,6 +64,13 @@ struct DevScan {
   const double* UA;      // [l(l+1)/2][G] packed chol(Stl - C_ll)
   const double* UA_invd; // [l][G] 1/diag
   const int* pat_pd;     // [G] 1 if Stl - C_ll is positive definite
+
+  // compact missing rows: U = union of the patterns' missing rows (sorted)
+  int nU;                // |U| (0: corrections read the genotypes directly)
+  const int* uidx;       // [n_pad] position of row k in U, -1 if absent
+  const double* XLu;     // [nU][l] covariate rows of U
+  const int* pu_off;     // [G+1] pattern g's rows are pu_list[pu_off[g]..pu_off[g+1])
+  const int* pu_list;    // positions in U
 
   // genotypes
   const void* Gm; int gdt; long long ldg;
@@ -75,6 +83,9 @@ struct DevScan {
   double* Str;           // [mc][l][r]
   double* varx;          // [mc][p]
   double* Ccorr;         // [mc][Q][G]
+  double* Xu;            // [nU][ldxu] imputed genotype values at the rows of U
+  double* pst;           // [mc][G][patsnp_width] per-(SNP, pattern) solve pieces
+  long long ldxu;        // >= mc*r
   const void* Cbuf;      // [mc*r][ldc]
 
   // outputs

@@ -12,15 +12,22 @@ void launch_verify_patterns(int t, int n, const int* rows, const int* rep_of, co
                             int* mismatch, hipStream_t st);
 void launch_pattern(const DevScan& d, const double* Stl_pk, int G, const int* pat_rep, double* UA,
                     double* UA_invd, int* pat_pd, hipStream_t st);
-void launch_trait_solve(const DevScan& d, int lb, double* zt, double* rss0, hipStream_t st);
+// zt = U_A^-T XL'y0, ay = U_A^-1 zt (covariate-only coefficients), rss0
+void launch_trait_solve(const DevScan& d, int lb, double* zt, double* ay, double* rss0, hipStream_t st);
 
 // SNP side (oagpu_kernels_snp.hip)
-// corr: also compute the missing-row corrections of every pattern (used
-// when the patterns' missing rows are few).
+// corr: also write the imputed values at the rows of U (d.Xu), read by
+// launch_corr (used when the patterns' missing rows are few).
 void launch_prep(const DevScan& d, int prec, void* Ahi, void* Alo, bool corr, hipStream_t st);
 void launch_snp_gram(const DevScan& d, int prec, int lb, hipStream_t st);
 void launch_mask(const DevScan& d, int lb, hipStream_t st);
+void launch_corr(const DevScan& d, int lb, hipStream_t st);
 void launch_epilogue(const DevScan& d, int prec, int lb, hipStream_t st);
+// Few patterns: per-(SNP, pattern) solve pieces (d.pst, patsnp_width
+// doubles each), then the per-pair epilogue that reads them.
+void launch_patsnp(const DevScan& d, int lb, hipStream_t st);
+void launch_epilogue_pst(const DevScan& d, int prec, int lb, hipStream_t st);
+int patsnp_width(int r, int lb);
 void launch_fallback(const DevScan& d, int lb, hipStream_t st);
 void launch_reset_counter(unsigned long long* c, int idx, hipStream_t st);
 

@@ -35,12 +35,16 @@ __device__ __forceinline__ double geno_value(const DevScan& d, int src, int env,
 }
 
 // ---------------------------------------------------------------------------
-// Operand preparation, SPW SNPs per block: mean imputation, interaction
+// Operand preparation, prep_spw<R>() SNPs per block: mean imputation, interaction
 // columns, the contraction operand A (column s*R+c, rows n..n_pad-1 zero),
 // Sbr = XR_s'XR_s and -- when the patterns' missing rows are few -- the
-// missing-row corrections C_lr, C_rr of every pattern.  Each thread handles
-// runs of 4 consecutive rows, so loads and operand stores are vectors.
-constexpr int PREP_SPW = 4;
+// imputed values at those rows (u-major, so the corrections kernel reads
+// them coalesced).  Each thread handles runs of 4 consecutive rows, so loads
+// and operand stores are vectors.
+// SNPs per block: about 4 output columns, so a thread's 4-row runs of every
+// column stay in registers at 5 waves per SIMD.
+template <int R>
+constexpr int prep_spw() { return R == 1 ? 4 : (R == 2 ? 2 : 1); }
 
 template <int OP>
 __device__ __forceinline__ void store_op4(void* Ahi, void* Alo, size_t o, const double* v) {
@@ -64,6 +68,7 @@ __device__ __forceinline__ void store_op4(void* Ahi, void* Alo, size_t o, const 
 template <int R, int OP, bool CORR>
 __global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo) {
   constexpr int RR = R * (R + 1) / 2;
+  constexpr int PREP_SPW = prep_spw<R>();
   constexpr int NV = PREP_SPW * R;
   const int tid = threadIdx.x;
   const int sb = blockIdx.x * PREP_SPW;
@@ -76,7 +81,6 @@ __global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo) {
     const int gs = d.s0 + (live[q] ? sb + q : sb);
     col_desc<R>(d, gs, src + q * R, env + q * R);
   }
-  const int l = d.l, Q = d.Q, G = d.G;
   // pass 1: NaN counts and sums per column
   double sum[NV], cnt[NV];
 #pragma unroll
@@ -116,6 +120,19 @@ __global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo) {
         }
         x[v][e] = val;
       }
+    if (CORR) {
+      // rows of U: their imputed values, u-major, for the corrections
+      const int4 ui = *reinterpret_cast<const int4*>(d.uidx + k0);
+      const int uu[4] = {ui.x, ui.y, ui.z, ui.w};
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (uu[e] >= 0) {
+          double* dst = d.Xu + (size_t)uu[e] * d.ldxu + (size_t)sb * R;
+#pragma unroll
+          for (int v = 0; v < NV; ++v)
+            if (live[v / R]) dst[v] = x[v][e];
+        }
+    }
 #pragma unroll
     for (int q = 0; q < PREP_SPW; ++q) {
       if (!live[q]) continue;
@@ -142,52 +159,6 @@ __global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo) {
       if (live[q])
 #pragma unroll
         for (int c = 0; c < R; ++c) d.col_mean[(size_t)(sb + q) * R + c] = mean[q * R + c];
-  }
-  if (CORR) {
-    // Missing-row corrections of every pattern, while the block's columns
-    // are still in cache: one wave per pattern, lanes over its rows, a wave
-    // reduction per term.  Ccorr layout [snp][term][pattern].
-    const int lane = tid & 63, wv = tid >> 6;
-    for (int g = wv; g < G; g += 4) {
-      const int cnt = d.pat_cnt[g];
-      const int* rows = d.rows + (size_t)d.pat_rep[g] * d.n;
-      for (int q = 0; q < PREP_SPW; ++q) {
-        if (!live[q]) continue;
-        const size_t base = (size_t)(sb + q) * Q;
-        for (int j = 0; j <= l; ++j) {
-          double a[RR];
-#pragma unroll
-          for (int e = 0; e < RR; ++e) a[e] = 0.0;
-          for (int i = lane; i < cnt; i += 64) {
-            const int k = rows[i];
-            double xv[R];
-#pragma unroll
-            for (int c = 0; c < R; ++c) {
-              double x = load_elem(d.Gm, d.gdt, (size_t)src[q * R + c] * d.ldg + k);
-              if (env[q * R + c] >= 0) x *= d.E[(size_t)env[q * R + c] * d.lde + k];
-              xv[c] = (x == x) ? x : mean[q * R + c];
-            }
-            if (j < l) {
-              const double xl = d.XL_rm[(size_t)k * l + j];
-#pragma unroll
-              for (int c = 0; c < R; ++c) a[c] += xl * xv[c];
-            } else {
-#pragma unroll
-              for (int b2 = 0; b2 < R; ++b2)
-#pragma unroll
-                for (int a2 = 0; a2 <= b2; ++a2) a[upk(a2, b2)] += xv[a2] * xv[b2];
-            }
-          }
-          const int nterm = (j < l) ? R : RR;
-#pragma unroll
-          for (int e = 0; e < RR; ++e) {
-            if (e >= nterm) break;
-            const double v = wave_sum(a[e]);
-            if (lane == 0) d.Ccorr[(base + (j < l ? j * R + e : l * R + e)) * G + g] = v;
-          }
-        }
-      }
-    }
   }
 }
 
@@ -380,6 +351,59 @@ __global__ __launch_bounds__(256) void k_mask(DevScan d) {
   }
 }
 
+// Corrections from the compact rows of U written by the operand pass: one
+// wave per pattern, one lane per SNP; the pattern's rows and covariate rows
+// are wave-uniform, the genotype values of 64 consecutive SNPs at one row are
+// contiguous.  Same sums as k_mask (src/Algorithm.cpp:1260-1315).
+template <int R, int LB>
+__global__ __launch_bounds__(256) void k_corr(DevScan d) {
+  const int lane = threadIdx.x & 63;
+  const int g = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + (threadIdx.x >> 6));
+  if (g >= d.G) return;
+  const int s = blockIdx.x * 64 + lane;
+  const int se = s < d.mc ? s : d.mc - 1;
+  const int l = d.l;
+  constexpr int RR = R * (R + 1) / 2;
+  double lr[LB][R], rr[RR];
+#pragma unroll
+  for (int j = 0; j < LB; ++j)
+#pragma unroll
+    for (int c = 0; c < R; ++c) lr[j][c] = 0.0;
+#pragma unroll
+  for (int e = 0; e < RR; ++e) rr[e] = 0.0;
+  const int beg = d.pu_off[g], end = d.pu_off[g + 1];
+#pragma unroll 4
+  for (int i = beg; i < end; ++i) {
+    const int u = d.pu_list[i];
+    const double* xr = d.Xu + (size_t)u * d.ldxu + (size_t)se * R;
+    double x[R];
+#pragma unroll
+    for (int c = 0; c < R; ++c) x[c] = xr[c];
+    const double* xl = d.XLu + (size_t)u * l;
+#pragma unroll
+    for (int j = 0; j < LB; ++j)
+      if (j < l) {
+        const double a = xl[j];
+#pragma unroll
+        for (int c = 0; c < R; ++c) lr[j][c] += a * x[c];
+      }
+#pragma unroll
+    for (int b = 0; b < R; ++b)
+#pragma unroll
+      for (int a = 0; a <= b; ++a) rr[upk(a, b)] += x[a] * x[b];
+  }
+  if (s < d.mc) {
+    const size_t base = (size_t)s * d.Q;
+#pragma unroll
+    for (int j = 0; j < LB; ++j)
+      if (j < l)
+#pragma unroll
+        for (int c = 0; c < R; ++c) d.Ccorr[(base + j * R + c) * d.G + g] = lr[j][c];
+#pragma unroll
+    for (int e = 0; e < RR; ++e) d.Ccorr[(base + l * R + e) * d.G + g] = rr[e];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // hpc_statistics for one (trait, SNP) pair (src/Algorithm.cpp:1034-1214):
 // R2, SE from the unmasked var_x scaled by (n - Ymiss)/n, t, the
@@ -498,6 +522,56 @@ __device__ double explicit_rss_thread(const DevScan& d, int y, int s, const doub
     acc += sy * sy;
   }
   return acc;
+}
+
+// Shared end of a pair: the compact coefficient vector, the hand-over to the
+// explicit residual pass, and the statistics.  b[0..LB) covariates,
+// b[LB..LB+R) SNP terms.
+template <int R, int LB>
+__device__ __forceinline__ void finish_pair(const DevScan& d, int y, int s, bool ok, const double* b,
+                                            const double* rb, double rss) {
+  constexpr int P = LB + R;
+  const int l = d.l;
+  // Compact coefficient vector: b[0..l) covariates, b[l..l+R) SNP terms.
+  double bc[P];
+#pragma unroll
+  for (int h = 0; h < P; ++h) bc[h] = 0.0;
+#pragma unroll
+  for (int h = 0; h < LB; ++h)
+    if (h < l) bc[h] = b[h];
+#pragma unroll
+  for (int c = 0; c < R; ++c) {
+#pragma unroll
+    for (int h = 0; h < P; ++h)
+      if (h == l + c) bc[h] = b[LB + c];
+  }
+  const bool fallback = !ok || d.yzero[y] > 0 || !(rss >= RSS_GUARD * d.yy[y]);
+  if (fallback) {
+    if (!ok) {   // sposv left the right-hand side in place
+#pragma unroll
+      for (int h = 0; h < P; ++h) {
+        if (h < l) bc[h] = d.rt[(size_t)h * d.ldt + y];
+      }
+#pragma unroll
+      for (int c = 0; c < R; ++c)
+#pragma unroll
+        for (int h = 0; h < P; ++h)
+          if (h == l + c) bc[h] = rb[c];
+    }
+    const unsigned long long slot = atomicAdd(&d.counters[CNT_FB_CHUNK], 1ull);
+    atomicAdd(&d.counters[CNT_FALLBACK], 1ull);
+    if (slot < (unsigned long long)d.fb_cap) {
+      FallbackEntry* f = d.fb + slot;
+      f->y = y;
+      f->s = s;
+      f->flags = ok ? 0 : 1;
+#pragma unroll
+      for (int h = 0; h < P; ++h) f->b[h] = bc[h];
+      return;
+    }
+    rss = explicit_rss_thread<R, LB>(d, y, s, bc);
+  }
+  stats_emit<P>(d, y, s, bc, rss);
 }
 
 // ---------------------------------------------------------------------------
@@ -685,46 +759,242 @@ __global__ __launch_bounds__(256) void k_epilogue(DevScan d) {
       rss += q;
     }
   }
-  // Compact coefficient vector: b[0..l) covariates, b[l..l+R) SNP terms.
-  double bc[P];
+  finish_pair<R, LB>(d, y, s, ok, b, rb, rss);
+}
+
+// ---------------------------------------------------------------------------
+// Few patterns (G << t): everything in the pair solve that depends only on
+// (SNP, pattern) is computed once per (SNP, pattern) instead of once per
+// pair.  With X = U_A^-T B_q, the Schur factor U_S, M = U_A^-1 X,
+// a_y = U_A^-1 zt (per trait), the covariate coefficients are
+// b_l = a_y - M b_r, and the reference's misplaced-correction terms are
+// 2 b_l' K b_r - b_r' H b_r with K, H from the corrections.
+template <int R, int LB>
+struct PatSnp {
+  static constexpr int RR = R * (R + 1) / 2;
+  static constexpr int X = 0, M = LB * R, K = 2 * LB * R, US = 3 * LB * R, H = 3 * LB * R + RR,
+                       OK = 3 * LB * R + 2 * RR, W = 3 * LB * R + 2 * RR + 1;
+};
+
+template <int R, int LB>
+__global__ __launch_bounds__(256) void k_patsnp(DevScan d) {
+  using T = PatSnp<R, LB>;
+  constexpr int RR = T::RR;
+  const int G = d.G, l = d.l;
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long long)d.mc * G) return;
+  const int s = (int)(idx / G), g = (int)(idx % G);
+  const int cnt = d.pat_cnt[g];
+  double X[LB][R], Clr[LB][R], Crr[RR];
 #pragma unroll
-  for (int h = 0; h < P; ++h) bc[h] = 0.0;
+  for (int j = 0; j < LB; ++j)
 #pragma unroll
-  for (int h = 0; h < LB; ++h)
-    if (h < l) bc[h] = b[h];
+    for (int c = 0; c < R; ++c) {
+      X[j][c] = (j < l) ? d.Str[((size_t)s * l + j) * R + c] : 0.0;
+      Clr[j][c] = 0.0;
+    }
 #pragma unroll
-  for (int c = 0; c < R; ++c) {
+  for (int e = 0; e < RR; ++e) Crr[e] = 0.0;
+  if (cnt > 0) {
+    // every column's correction is subtracted from the first SNP column
+    // (src/Algorithm.cpp:1390-1396)
+    const size_t base = (size_t)s * d.Q;
 #pragma unroll
-    for (int h = 0; h < P; ++h)
-      if (h == l + c) bc[h] = b[LB + c];
-  }
-  const bool fallback = !ok || d.yzero[y] > 0 || !(rss >= RSS_GUARD * d.yy[y]);
-  if (fallback) {
-    if (!ok) {   // sposv left the right-hand side in place
+    for (int j = 0; j < LB; ++j)
+      if (j < l) {
+        double tot = 0.0;
 #pragma unroll
-      for (int h = 0; h < P; ++h) {
-        if (h < l) bc[h] = d.rt[(size_t)h * d.ldt + y];
+        for (int c = 0; c < R; ++c) {
+          const double v = d.Ccorr[(base + j * R + c) * G + g];
+          tot += v;
+          Clr[j][c] = v;
+        }
+        X[j][0] -= tot;
       }
 #pragma unroll
-      for (int c = 0; c < R; ++c)
-#pragma unroll
-        for (int h = 0; h < P; ++h)
-          if (h == l + c) bc[h] = rb[c];
-    }
-    const unsigned long long slot = atomicAdd(&d.counters[CNT_FB_CHUNK], 1ull);
-    atomicAdd(&d.counters[CNT_FALLBACK], 1ull);
-    if (slot < (unsigned long long)d.fb_cap) {
-      FallbackEntry* f = d.fb + slot;
-      f->y = y;
-      f->s = s;
-      f->flags = ok ? 0 : 1;
-#pragma unroll
-      for (int h = 0; h < P; ++h) f->b[h] = bc[h];
-      return;
-    }
-    rss = explicit_rss_thread<R, LB>(d, y, s, bc);
+    for (int e = 0; e < RR; ++e) Crr[e] = d.Ccorr[(base + l * R + e) * G + g];
   }
-  stats_emit<P>(d, y, s, bc, rss);
+  // D_q: Sbr corrected only in rows a < min(l, r) (src/Algorithm.cpp:1300)
+  double Sc[RR];
+#pragma unroll
+  for (int b2 = 0; b2 < R; ++b2)
+#pragma unroll
+    for (int a = 0; a <= b2; ++a)
+      Sc[upk(a, b2)] = d.Sbr[(size_t)s * RR + upk(a, b2)] - ((a < l) ? Crr[upk(a, b2)] : 0.0);
+  // X = U_A^-T B_q
+#pragma unroll
+  for (int j = 0; j < LB; ++j)
+    if (j < l) {
+      const double inv = d.UA_invd[(size_t)j * G + g];
+#pragma unroll
+      for (int c = 0; c < R; ++c) {
+        double v = X[j][c];
+#pragma unroll
+        for (int i = 0; i < j; ++i) v -= d.UA[(size_t)upk(i, j) * G + g] * X[i][c];
+        X[j][c] = v * inv;
+      }
+    }
+  // Schur complement and its Cholesky factor
+#pragma unroll
+  for (int b2 = 0; b2 < R; ++b2)
+#pragma unroll
+    for (int a = 0; a <= b2; ++a) {
+      double v = Sc[upk(a, b2)];
+#pragma unroll
+      for (int j = 0; j < LB; ++j) v -= X[j][a] * X[j][b2];
+      Sc[upk(a, b2)] = v;
+    }
+  bool ok = d.pat_pd[g] != 0;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    double ajj = Sc[upk(j, j)];
+#pragma unroll
+    for (int k = 0; k < j; ++k) ajj -= Sc[upk(k, j)] * Sc[upk(k, j)];
+    if (!(ajj > 0.0)) ok = false;
+    ajj = sqrt(ajj);
+    Sc[upk(j, j)] = ajj;
+#pragma unroll
+    for (int i = j + 1; i < R; ++i) {
+      double aji = Sc[upk(j, i)];
+#pragma unroll
+      for (int k = 0; k < j; ++k) aji -= Sc[upk(k, j)] * Sc[upk(k, i)];
+      Sc[upk(j, i)] = aji / ajj;
+    }
+  }
+  double* out = d.pst + (size_t)idx * T::W;
+  // M = U_A^-1 X
+  double Mv[LB][R];
+#pragma unroll
+  for (int j = LB - 1; j >= 0; --j) {
+#pragma unroll
+    for (int c = 0; c < R; ++c) Mv[j][c] = 0.0;
+    if (j < l) {
+      const double inv = d.UA_invd[(size_t)j * G + g];
+#pragma unroll
+      for (int c = 0; c < R; ++c) {
+        double v = X[j][c];
+#pragma unroll
+        for (int i = j + 1; i < LB; ++i)
+          if (i < l) v -= d.UA[(size_t)upk(j, i) * G + g] * Mv[i][c];
+        Mv[j][c] = v * inv;
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < LB; ++j)
+    if (j < l)
+#pragma unroll
+      for (int c = 0; c < R; ++c) {
+        out[T::X + j * R + c] = X[j][c];
+        out[T::M + j * R + c] = Mv[j][c];
+        double k = 0.0;
+        if (R > 1) {
+          if (c == 0) {
+#pragma unroll
+            for (int c2 = 1; c2 < R; ++c2) k += Clr[j][c2];
+          } else {
+            k = -Clr[j][c];
+          }
+        }
+        out[T::K + j * R + c] = k;
+      }
+#pragma unroll
+  for (int b2 = 0; b2 < R; ++b2)
+#pragma unroll
+    for (int a = 0; a <= b2; ++a) {
+      out[T::US + upk(a, b2)] = Sc[upk(a, b2)];
+      out[T::H + upk(a, b2)] = (a >= l) ? (a == b2 ? 1.0 : 2.0) * Crr[upk(a, b2)] : 0.0;
+    }
+  out[T::OK] = ok ? 1.0 : 0.0;
+}
+
+// One thread per (trait, SNP) pair, reading the (SNP, pattern) table.
+template <int R, int LB, typename CT>
+__global__ __launch_bounds__(256) void k_epilogue_pst(DevScan d) {
+  using T = PatSnp<R, LB>;
+  constexpr int RR = T::RR;
+  constexpr int P = LB + R;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int s = blockIdx.y * 4 + w;
+  if (s >= d.mc) return;
+  const int y = blockIdx.x * 64 + lane;
+  if (y >= d.t) return;
+  const int l = d.l;
+  const int g = d.ypat[y];
+  const double* tb = d.pst + ((size_t)s * d.G + g) * T::W;
+  const CT* C = reinterpret_cast<const CT*>(d.Cbuf);
+  double rb[R];
+#pragma unroll
+  for (int c = 0; c < R; ++c) rb[c] = (double)C[(size_t)(s * R + c) * d.ldc + y];
+  const bool ok = tb[T::OK] != 0.0;
+  double b[P];
+#pragma unroll
+  for (int h = 0; h < P; ++h) b[h] = 0.0;
+  double rss = 0.0;
+  if (ok) {
+    double z[R];
+#pragma unroll
+    for (int c = 0; c < R; ++c) z[c] = rb[c];
+#pragma unroll
+    for (int j = 0; j < LB; ++j)
+      if (j < l) {
+        const double ztj = d.zt[(size_t)j * d.ldt + y];
+#pragma unroll
+        for (int c = 0; c < R; ++c) z[c] -= tb[T::X + j * R + c] * ztj;
+      }
+    double us[RR];
+#pragma unroll
+    for (int e = 0; e < RR; ++e) us[e] = tb[T::US + e];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {   // U_S' z = num
+      double v = z[i];
+#pragma unroll
+      for (int k = 0; k < i; ++k) v -= us[upk(k, i)] * z[k];
+      z[i] = v / us[upk(i, i)];
+    }
+    double zz = 0.0;
+#pragma unroll
+    for (int c = 0; c < R; ++c) zz += z[c] * z[c];
+    double br[R];
+#pragma unroll
+    for (int i = R - 1; i >= 0; --i) {   // U_S b_r = z
+      double v = z[i];
+#pragma unroll
+      for (int k = i + 1; k < R; ++k) v -= us[upk(i, k)] * br[k];
+      br[i] = v / us[upk(i, i)];
+    }
+    rss = d.rss0[y] - zz;
+#pragma unroll
+    for (int c = 0; c < R; ++c) b[LB + c] = br[c];
+    const bool quirk = (R > 1) && d.pat_cnt[g] > 0;
+    const bool need_bl = d.need_bl || quirk || d.yzero[y] > 0 || !(rss >= RSS_GUARD * d.yy[y]);
+    if (need_bl) {
+      double q = 0.0;
+#pragma unroll
+      for (int j = 0; j < LB; ++j)
+        if (j < l) {
+          double v = d.ay[(size_t)j * d.ldt + y];
+#pragma unroll
+          for (int c = 0; c < R; ++c) v -= tb[T::M + j * R + c] * br[c];
+          b[j] = v;
+          if (quirk) {
+            double kv = 0.0;
+#pragma unroll
+            for (int c = 0; c < R; ++c) kv += tb[T::K + j * R + c] * br[c];
+            q += 2.0 * v * kv;
+          }
+        }
+      if (quirk) {
+#pragma unroll
+        for (int b2 = 0; b2 < R; ++b2)
+#pragma unroll
+          for (int a = 0; a <= b2; ++a) q -= tb[T::H + upk(a, b2)] * br[a] * br[b2];
+        rss += q;
+      }
+    }
+  }
+  finish_pair<R, LB>(d, y, s, ok, b, rb, rss);
 }
 
 // Explicit residual pass for the pairs the epilogue handed over: one wave
@@ -777,7 +1047,7 @@ __global__ void k_reset_counter(unsigned long long* c, int idx) { c[idx] = 0ull;
 // launchers
 template <int R>
 static void prep_r(const DevScan& d, int prec, void* Ahi, void* Alo, bool corr, hipStream_t st) {
-  dim3 grid((d.mc + PREP_SPW - 1) / PREP_SPW), block(256);
+  dim3 grid((d.mc + prep_spw<R>() - 1) / prep_spw<R>()), block(256);
 #define OA_PR(OP_)                                                                      \
   if (corr) hipLaunchKernelGGL((k_prep<R, OP_, true>), grid, block, 0, st, d, Ahi, Alo); \
   else hipLaunchKernelGGL((k_prep<R, OP_, false>), grid, block, 0, st, d, Ahi, Alo);
@@ -822,6 +1092,13 @@ void launch_mask(const DevScan& d, int lb, hipStream_t st) {
 #undef OA_MK
 }
 
+void launch_corr(const DevScan& d, int lb, hipStream_t st) {
+  dim3 grid((d.mc + 63) / 64, (d.G + 3) / 4), block(256);
+#define OA_CR(R_, LB_) hipLaunchKernelGGL((k_corr<R_, LB_>), grid, block, 0, st, d);
+  OA_DISPATCH_RLB(0, 0, OA_CR)
+#undef OA_CR
+}
+
 void launch_epilogue(const DevScan& d, int prec, int lb, hipStream_t st) {
   dim3 grid((d.t + 63) / 64, (d.mc + 3) / 4), block(256);
 #define OA_EP(R_, LB_)                                                                          \
@@ -829,6 +1106,25 @@ void launch_epilogue(const DevScan& d, int prec, int lb, hipStream_t st) {
   else hipLaunchKernelGGL((k_epilogue<R_, LB_, float>), grid, block, 0, st, d);
   OA_DISPATCH_RLB(0, 0, OA_EP)
 #undef OA_EP
+}
+
+void launch_patsnp(const DevScan& d, int lb, hipStream_t st) {
+  const long long nthr = (long long)d.mc * d.G;
+  dim3 grid((unsigned)((nthr + 255) / 256)), block(256);
+#define OA_PS(R_, LB_) hipLaunchKernelGGL((k_patsnp<R_, LB_>), grid, block, 0, st, d);
+  OA_DISPATCH_RLB(0, 0, OA_PS)
+#undef OA_PS
+}
+
+int patsnp_width(int r, int lb) { return 3 * lb * r + r * (r + 1) + 1; }
+
+void launch_epilogue_pst(const DevScan& d, int prec, int lb, hipStream_t st) {
+  dim3 grid((d.t + 63) / 64, (d.mc + 3) / 4), block(256);
+#define OA_EQ(R_, LB_)                                                                              \
+  if (prec == PREC_FP64) hipLaunchKernelGGL((k_epilogue_pst<R_, LB_, double>), grid, block, 0, st, d); \
+  else hipLaunchKernelGGL((k_epilogue_pst<R_, LB_, float>), grid, block, 0, st, d);
+  OA_DISPATCH_RLB(0, 0, OA_EQ)
+#undef OA_EQ
 }
 
 void launch_fallback(const DevScan& d, int lb, hipStream_t st) {

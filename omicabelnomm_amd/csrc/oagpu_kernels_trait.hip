@@ -162,7 +162,7 @@ __global__ __launch_bounds__(256) void k_pattern(DevScan d, const double* Stl_pk
 // One thread per trait: zt = U_A^-T XL'y0 and RSS0 = y0'y0 - |zt|^2 (the
 // covariate-only residual, from which every pair's RSS is a downdate).
 template <int LB>
-__global__ void k_trait_solve(DevScan d, double* zt, double* rss0) {
+__global__ void k_trait_solve(DevScan d, double* zt, double* ay, double* rss0) {
   const int y = blockIdx.x * blockDim.x + threadIdx.x;
   if (y >= d.t) return;
   const int g = d.ypat[y];
@@ -182,6 +182,18 @@ __global__ void k_trait_solve(DevScan d, double* zt, double* rss0) {
     }
   }
   rss0[y] = d.yy[y] - ss;
+  // a = U_A^-1 z: the covariate-only coefficients
+#pragma unroll
+  for (int j = LB - 1; j >= 0; --j) {
+    if (j < l) {
+      double v = z[j];
+#pragma unroll
+      for (int i = j + 1; i < LB; ++i)
+        if (i < l) v -= d.UA[(size_t)upk(j, i) * G + g] * z[i];
+      z[j] = v * d.UA_invd[(size_t)j * G + g];
+      ay[(size_t)j * d.ldt + y] = z[j];
+    }
+  }
 }
 
 // ---- host launchers ----
@@ -209,12 +221,12 @@ void launch_pattern(const DevScan& d, const double* Stl_pk, int G, const int* pa
   hipLaunchKernelGGL(k_pattern, dim3(G), dim3(256), sh, st, d, Stl_pk, G, pat_rep, UA, UA_invd, pat_pd);
 }
 
-void launch_trait_solve(const DevScan& d, int lb, double* zt, double* rss0, hipStream_t st) {
+void launch_trait_solve(const DevScan& d, int lb, double* zt, double* ay, double* rss0, hipStream_t st) {
   dim3 grid((d.t + 127) / 128), block(128);
   switch (lb) {
-    case 8: hipLaunchKernelGGL(k_trait_solve<8>, grid, block, 0, st, d, zt, rss0); break;
-    case 16: hipLaunchKernelGGL(k_trait_solve<16>, grid, block, 0, st, d, zt, rss0); break;
-    default: hipLaunchKernelGGL(k_trait_solve<32>, grid, block, 0, st, d, zt, rss0); break;
+    case 8: hipLaunchKernelGGL(k_trait_solve<8>, grid, block, 0, st, d, zt, ay, rss0); break;
+    case 16: hipLaunchKernelGGL(k_trait_solve<16>, grid, block, 0, st, d, zt, ay, rss0); break;
+    default: hipLaunchKernelGGL(k_trait_solve<32>, grid, block, 0, st, d, zt, ay, rss0); break;
   }
 }
 

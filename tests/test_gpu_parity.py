@@ -65,6 +65,20 @@ def test_chunked_and_device_inputs(oracle_mod):
     assert g.stats["n_chunks"] == 7
 
 
+def test_chunked_shared_patterns(oracle_mod):
+    """Few shared missing-row patterns (the compact correction path, whose
+    row buffer is strided by the chunk width) over ragged chunks, both
+    genotype element types."""
+    for gdt in (torch.float32, torch.float64):
+        data = synth.make(n=410, l=5, m=53, t=40, interaction=True, miss="groups", pattern_groups=5,
+                          pattern_rate=0.03, planted=8, seed=21, g_dtype=gdt)
+        print("case chunked/groups", gdt, flush=True)
+        g = run_gpu(data, "fp64", max_snps=9, disp_cov=True, min_r2_store=-10)
+        o = run_oracle(oracle_mod, data, disp_cov=True, min_r2_store=-10)
+        compare_fp64(g, o)
+        assert g.stats["n_chunks"] == 6
+
+
 def test_observed_zero_and_near_perfect_fit(oracle_mod):
     """Observed phenotype values of exactly 0 (the reference's RSS skips
     them) and near-perfect fits both take the explicit residual pass."""
