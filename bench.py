@@ -155,7 +155,7 @@ def main():
     prec = None
     if args.fp64_steps > 0 and args.precision != "fp64":
         sc = make_scanner(data, "fp64", args.p_store)
-        dt64, res64, gemm64_ms, chunks64 = timed(sc, data, args.fp64_steps, 0)
+        dt64, res64, gemm64_ms, chunks64 = timed(sc, data, args.fp64_steps, 1)
         variant64 = sc.gemm_variant()
         sc.close()
         fp64 = {"value": tests / dt64, "ms_per_step": dt64 * 1e3,

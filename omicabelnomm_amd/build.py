@@ -20,7 +20,10 @@ LIB = os.path.join(HERE, "liboagpu.so")
 HOSTLIB = os.path.join(HERE, "libomicabelnomm.so")
 CLI = os.path.join(HERE, "omicabelnomm")
 
-HIP_SOURCES = ["oagpu_api.hip", "oagpu_kernels_trait.hip", "oagpu_kernels_snp.hip", "oagpu_gemm.hip"]
+HIP_SOURCES = ["oagpu_api.hip", "oagpu_kernels_trait.hip", "oagpu_kernels_snp.hip", "oagpu_gemm.hip",
+               "oagpu_gemm_lib.hip"]
+# library GEMM candidates of the contraction autotuner
+HIP_LIBS = ["-L/opt/rocm/lib", "-lhipblaslt", "-Wl,-rpath,/opt/rocm/lib"]
 HOST_SOURCES = ["host/databel.cpp", "host/writers.cpp", "host/driver.cpp"]
 CLI_SOURCES = ["host/main.cpp"]
 
@@ -54,7 +57,7 @@ def build(verbose: bool = True, cli: bool = True) -> str:
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, HIP_SOURCES))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + HIP_LIBS)
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)

@@ -26,6 +26,9 @@ struct oagpu_ctx {
   std::string err;
   int n = 0, n_pad = 0, l = 0, r = 0, p = 0, Q = 0, lb = 0;
   size_t op_size = 8, c_size = 8;
+  // Row stride (elements) of the contraction operands A and B.  Split-BF16
+  // rows interleave the two halves: A rows [hi | lo], B rows [lo | hi].
+  long long ldop = 0;
 
   // covariates
   bool have_cov = false;
@@ -47,7 +50,8 @@ struct oagpu_ctx {
   long long ldy = 0;
   void* Y_own = nullptr;
   size_t Y_own_bytes = 0;
-  void *Bhi = nullptr, *Blo = nullptr;
+  void* Bbuf = nullptr;
+  void *Bhi = nullptr, *Blo = nullptr;   // views into Bbuf
   int* rows = nullptr;
   int *ymiss = nullptr, *yzero = nullptr, *ypat = nullptr;
   double *yy = nullptr, *sst = nullptr, *rss0 = nullptr, *rt = nullptr, *zt = nullptr, *ay = nullptr;
@@ -99,14 +103,15 @@ struct oagpu_ctx {
 
   // chunk buffers
   int mc_cap = 0;
-  void *Ahi = nullptr, *Alo = nullptr, *Cbuf = nullptr;
+  void* Abuf = nullptr;
+  void *Ahi = nullptr, *Alo = nullptr, *Cbuf = nullptr;   // Ahi, Alo: views into Abuf
   double *col_mean = nullptr, *Sbr = nullptr, *Str = nullptr, *varx = nullptr, *Ccorr = nullptr;
   double* Xu = nullptr;
   long long ldxu = 0;
   // few patterns per trait block: per-(SNP, pattern) solve pieces
   bool use_pst = false;
   double* pst = nullptr;
-  size_t Ahi_cap = 0, Alo_cap = 0, C_cap = 0, Ccorr_cap = 0, Xu_cap = 0, pst_cap = 0;
+  size_t A_cap = 0, C_cap = 0, Ccorr_cap = 0, Xu_cap = 0, pst_cap = 0;
 
   // outputs
   oagpu_hit* hits = nullptr;
@@ -236,10 +241,15 @@ static double min_singular(const std::vector<double>& S, int l) {
   return mn;
 }
 
+// Row stride of the contraction output: traits then covariates, padded to
+// 64 columns (the padding columns are computed from zero or stale B rows and
+// never read), so library GEMMs see aligned shapes.
+static int out_cols(const oagpu_ctx* c) { return (c->t + c->l + 63) / 64 * 64; }
+
 static DevScan make_dev(const oagpu_ctx* c) {
   DevScan d{};
-  d.n = c->n; d.n_pad = c->n_pad; d.l = c->l; d.r = c->r; d.p = c->p; d.Q = c->Q;
-  d.t = c->t; d.ldt = c->ldt; d.ldc = c->t + c->l; d.G = c->G;
+  d.n = c->n; d.n_pad = c->n_pad; d.ldop = c->ldop; d.l = c->l; d.r = c->r; d.p = c->p; d.Q = c->Q;
+  d.t = c->t; d.ldt = c->ldt; d.ldc = out_cols(c); d.G = c->G;
   d.start_h = c->prm.disp_cov ? 1 : c->l;
   d.disp_cov = c->prm.disp_cov; d.store_all = c->prm.store_all; d.need_bl = c->prm.disp_cov;
   d.min_r2_store = (double)c->prm.min_r2_store;
@@ -256,6 +266,7 @@ static DevScan make_dev(const oagpu_ctx* c) {
   d.uidx = c->uidx; d.XLu = c->XLu; d.pu_off = c->pu_off; d.pu_list = c->pu_list;
   d.Xu = c->Xu; d.ldxu = c->ldxu; d.pst = c->pst;
   d.Gm = c->Gm; d.gdt = c->gdt; d.ldg = c->ldg; d.col_src = c->col_src; d.col_env = c->col_env;
+  d.g_vec = c->Gm && ((uintptr_t)c->Gm % 16 == 0) && (c->ldg % 4 == 0);
   d.col_mean = c->col_mean; d.Sbr = c->Sbr; d.Str = c->Str; d.varx = c->varx; d.Ccorr = c->Ccorr;
   d.Cbuf = c->Cbuf;
   d.hits = c->hits; d.hit_cap = c->hit_cap; d.counters = c->counters;
@@ -298,6 +309,7 @@ extern "C" int oagpu_create(const oagpu_params* prm, oagpu_ctx** out) {
   c->lb = l <= 8 ? 8 : (l <= 16 ? 16 : 32);
   c->op_size = prm->precision == OAGPU_FP64 ? 8 : (prm->precision == OAGPU_FP32 ? 4 : 2);
   c->c_size = prm->precision == OAGPU_FP64 ? 8 : 4;
+  c->ldop = (long long)c->n_pad * (prm->precision == OAGPU_BF16X3 ? 2 : 1);
   c->hit_cap = prm->max_hits > 0 ? prm->max_hits : (1ll << 22);
   c->fb_cap = 1 << 16;
   HIPCK(c, hipMalloc(&c->hits, sizeof(oagpu_hit) * (size_t)c->hit_cap));
@@ -307,11 +319,13 @@ extern "C" int oagpu_create(const oagpu_params* prm, oagpu_ctx** out) {
   const int T = prm->max_traits;
   c->ldt = (T + 63) / 64 * 64;
   const size_t brows = (size_t)((T + l + 127) / 128 * 128);
-  HIPCK(c, hipMalloc(&c->Bhi, brows * c->n_pad * c->op_size));
-  HIPCK(c, hipMemset(c->Bhi, 0, brows * c->n_pad * c->op_size));
+  HIPCK(c, hipMalloc(&c->Bbuf, brows * c->ldop * c->op_size));
+  HIPCK(c, hipMemset(c->Bbuf, 0, brows * c->ldop * c->op_size));
   if (prm->precision == OAGPU_BF16X3) {
-    HIPCK(c, hipMalloc(&c->Blo, brows * c->n_pad * c->op_size));
-    HIPCK(c, hipMemset(c->Blo, 0, brows * c->n_pad * c->op_size));
+    c->Blo = c->Bbuf;
+    c->Bhi = static_cast<uint16_t*>(c->Bbuf) + c->n_pad;
+  } else {
+    c->Bhi = c->Bbuf;
   }
   HIPCK(c, hipMalloc(&c->rows, sizeof(int) * (size_t)T * n));
   HIPCK(c, hipMalloc(&c->ymiss, sizeof(int) * T));
@@ -346,9 +360,9 @@ extern "C" void oagpu_destroy(oagpu_ctx* c) {
                         c->pd_varx};
   for (void* b : dense_bufs) dfree(b);
   void* bufs[] = {c->XL_cm, c->XL_rm, c->U0, c->Stl_inv_d, c->Stl_pk, c->XLop_hi, c->XLop_lo, c->E,
-                  c->Y_own, c->Bhi, c->Blo, c->rows, c->ymiss, c->yzero, c->ypat, c->yy, c->sst,
+                  c->Y_own, c->Bbuf, c->rows, c->ymiss, c->yzero, c->ypat, c->yy, c->sst,
                   c->rss0, c->rt, c->zt, c->ay, c->hash, c->scratch_i, c->pat_cnt, c->pat_rep, c->pat_pd,
-                  c->UA, c->UA_invd, c->Ahi, c->Alo, c->Cbuf,
+                  c->UA, c->UA_invd, c->Abuf, c->Cbuf,
                   c->col_mean, c->Sbr, c->Str, c->varx, c->Ccorr, c->hits, c->counters, c->fb,
                   c->uidx, c->pu_off, c->pu_list, c->XLu, c->Xu, c->pst};
   for (void* b : bufs) dfree(b);
@@ -458,16 +472,18 @@ extern "C" int oagpu_set_interactions(oagpu_ctx* c, const double* E, long long l
   HIPCK(c, hipSetDevice(c->prm.device));
   dfree(c->E);
   c->E = nullptr;
-  std::vector<double> e((size_t)n_env * c->n);
+  // rows padded to n_pad with zeros, so kernels load 4-row runs as vectors
+  const size_t np = (size_t)c->n_pad;
+  std::vector<double> e((size_t)n_env * np, 0.0);
   for (int j = 0; j < n_env; ++j)
     for (int k = 0; k < c->n; ++k) {
-      e[(size_t)j * c->n + k] = E[(size_t)j * ld + k];
-      if (std::isnan(e[(size_t)j * c->n + k]))
+      e[(size_t)j * np + k] = E[(size_t)j * ld + k];
+      if (std::isnan(e[(size_t)j * np + k]))
         return fail(c, OAGPU_ERR_ARG, "interaction covariates contain NaN (rows must be excluded first)");
     }
   HIPCK(c, hipMalloc(&c->E, sizeof(double) * e.size()));
   HIPCK(c, hipMemcpy(c->E, e.data(), sizeof(double) * e.size(), hipMemcpyHostToDevice));
-  c->lde = c->n;
+  c->lde = (long long)np;
   c->n_env = n_env;
   return OAGPU_OK;
 }
@@ -504,10 +520,12 @@ extern "C" int oagpu_load_traits(oagpu_ctx* c, const void* Y, int dtype, int whe
                 c->hash, c->st);
   HIPCK(c, hipGetLastError());
   // covariate rows of B after the traits
-  const size_t rowb = (size_t)c->n_pad * c->op_size;
-  HIPCK(c, hipMemcpyAsync((char*)c->Bhi + rowb * t, c->XLop_hi, rowb * l, hipMemcpyDeviceToDevice, c->st));
+  const size_t rowb = (size_t)c->n_pad * c->op_size, pitch = (size_t)c->ldop * c->op_size;
+  HIPCK(c, hipMemcpy2DAsync((char*)c->Bhi + pitch * t, pitch, c->XLop_hi, rowb, rowb, l, hipMemcpyDeviceToDevice,
+                            c->st));
   if (c->Blo)
-    HIPCK(c, hipMemcpyAsync((char*)c->Blo + rowb * t, c->XLop_lo, rowb * l, hipMemcpyDeviceToDevice, c->st));
+    HIPCK(c, hipMemcpy2DAsync((char*)c->Blo + pitch * t, pitch, c->XLop_lo, rowb, rowb, l,
+                              hipMemcpyDeviceToDevice, c->st));
 
   // Group traits by missing-row pattern: key = (count, 128-bit hash),
   // verified row by row on the device.
@@ -702,8 +720,8 @@ static int ensure_chunk(oagpu_ctx* c, int mc) {
   const int r = c->r, l = c->l, p = c->p, G = c->G, Q = c->Q;
   const size_t pad = (size_t)gemm_row_pad();
   const size_t arows = ((size_t)mc * r + pad - 1) / pad * pad;
-  const size_t abytes = arows * c->n_pad * c->op_size;
-  const size_t cbytes = (size_t)mc * r * (c->t + l) * c->c_size;
+  const size_t abytes = arows * c->ldop * c->op_size;
+  const size_t cbytes = (size_t)mc * r * out_cols(c) * c->c_size;
   const size_t qbytes = sizeof(double) * (size_t)mc * Q * std::max(G, 1);
   c->ldxu = ((long long)mc * r + 7) / 8 * 8;
   const size_t xubytes = c->compact_corr ? sizeof(double) * (size_t)c->nU * c->ldxu : 0;
@@ -715,8 +733,9 @@ static int ensure_chunk(oagpu_ctx* c, int mc) {
     HIPCK(c, hipMalloc(&c->varx, sizeof(double) * (size_t)mc * p));
     c->mc_cap = mc;
   }
-  HIPCK(c, grow(&c->Ahi, &c->Ahi_cap, abytes));
-  if (c->prm.precision == OAGPU_BF16X3) HIPCK(c, grow(&c->Alo, &c->Alo_cap, abytes));
+  HIPCK(c, grow(&c->Abuf, &c->A_cap, abytes));
+  c->Ahi = c->Abuf;
+  c->Alo = c->prm.precision == OAGPU_BF16X3 ? static_cast<uint16_t*>(c->Abuf) + c->n_pad : nullptr;
   HIPCK(c, grow(&c->Cbuf, &c->C_cap, cbytes));
   HIPCK(c, grow((void**)&c->Ccorr, &c->Ccorr_cap, qbytes));
   if (xubytes) HIPCK(c, grow((void**)&c->Xu, &c->Xu_cap, xubytes));
@@ -736,13 +755,13 @@ static int scan_enqueue(oagpu_ctx* c, bool want_dense) {
   c->use_pst = (long long)c->G * 4 <= t;
   // Chunk size: bounded by max_snps and by ~12 GiB of chunk buffers.
   const size_t per_snp = (size_t)r * c->n_pad * c->op_size * (c->prm.precision == OAGPU_BF16X3 ? 2 : 1) +
-                         (size_t)r * (t + l) * c->c_size + sizeof(double) * (size_t)c->Q * std::max(c->G, 1) +
+                         (size_t)r * out_cols(c) * c->c_size + sizeof(double) * (size_t)c->Q * std::max(c->G, 1) +
                          sizeof(double) * (size_t)(r + r * (r + 1) / 2 + l * r + p) +
                          (c->compact_corr ? sizeof(double) * (size_t)c->nU * r : 0) +
                          (c->use_pst ? sizeof(double) * (size_t)c->G * patsnp_width(r, c->lb) : 0);
   long long mc = c->prm.max_snps > 0 ? c->prm.max_snps : (long long)((12ull << 30) / per_snp);
   mc = std::max(1ll, std::min(mc, (long long)m));
-  mc = std::min(mc, (long long)(0x7fffffff / std::max(1, r * (t + l))));
+  mc = std::min(mc, (long long)(0x7fffffff / std::max(1, r * out_cols(c))));
   int rc = ensure_chunk(c, (int)mc);
   if (rc) return rc;
 
@@ -781,7 +800,8 @@ static int scan_enqueue(oagpu_ctx* c, bool want_dense) {
     HIPCK(c, hipEventRecord(E[0], c->st));
     launch_prep(d, c->prm.precision, c->Ahi, c->Alo, c->compact_corr, c->st);
     HIPCK(c, hipEventRecord(E[1], c->st));
-    launch_gemm(c->prm.precision, c->Ahi, c->Alo, c->Bhi, c->Blo, c->Cbuf, mcc * r, t + l, c->n_pad, t + l,
+    launch_gemm(c->prm.precision, c->Ahi, c->Alo, c->Bhi, c->Blo, c->Cbuf, mcc * r, out_cols(c), c->n_pad,
+                (int)c->ldop, (int)c->ldop, out_cols(c),
                 c->st);
     HIPCK(c, hipEventRecord(E[2], c->st));
     launch_snp_gram(d, c->prm.precision, c->lb, c->st);
@@ -798,7 +818,7 @@ static int scan_enqueue(oagpu_ctx* c, bool want_dense) {
     HIPCK(c, hipEventRecord(E[6], c->st));
     HIPCK(c, hipGetLastError());
   }
-  c->gemm_variant = gemm_variant_name(c->prm.precision, (int)std::min<long long>(mc, m) * r, t + l, c->n_pad);
+  c->gemm_variant = gemm_variant_name(c->prm.precision, (int)std::min<long long>(mc, m) * r, out_cols(c), c->n_pad);
   // the genotype slot may be overwritten once everything above has run
   HIPCK(c, hipEventRecord(c->free_ev[c->gslot], c->st));
   c->free_valid[c->gslot] = true;

@@ -26,6 +26,30 @@ __device__ __forceinline__ double load_elem(const void* base, int dt, size_t idx
   return (double)(float)reinterpret_cast<const _Float16*>(base)[idx];
 }
 
+// Four consecutive elements idx..idx+3.  vec: idx is a multiple of 4 and the
+// base is 16-byte aligned, so one vector load covers them.
+__device__ __forceinline__ void load_elem4(const void* base, int dt, bool vec, size_t idx, double* o) {
+  if (vec) {
+    if (dt == DT_F32) {
+      const float4 v = *reinterpret_cast<const float4*>(reinterpret_cast<const float*>(base) + idx);
+      o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+    } else if (dt == DT_F64) {
+      const double2 a = *reinterpret_cast<const double2*>(reinterpret_cast<const double*>(base) + idx);
+      const double2 b = *reinterpret_cast<const double2*>(reinterpret_cast<const double*>(base) + idx + 2);
+      o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+    } else {
+      const uint2 u = *reinterpret_cast<const uint2*>(reinterpret_cast<const _Float16*>(base) + idx);
+      _Float16 h[4];
+      __builtin_memcpy(h, &u, sizeof(h));
+#pragma unroll
+      for (int e = 0; e < 4; ++e) o[e] = (double)(float)h[e];
+    }
+    return;
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) o[e] = load_elem(base, dt, idx + e);
+}
+
 // float -> bf16 bits, round to nearest even (finite inputs).
 __device__ __forceinline__ uint16_t bf16_bits_rn(float f) {
   uint32_t u = __float_as_uint(f);

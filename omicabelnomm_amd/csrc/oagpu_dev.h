@@ -28,6 +28,7 @@ enum {
 struct DevScan {
   // shapes
   int n, n_pad, l, r, p, Q;   // Q = l*r + r*(r+1)/2 correction terms
+  long long ldop;             // row stride of the contraction operands (elements)
   int t, ldt;                 // traits, stride of per-trait arrays
   int ldc;                    // row stride of the contraction output (>= t + l)
   int G;                      // missing-value patterns
@@ -42,7 +43,7 @@ struct DevScan {
   const double* Stl_inv_diag;  // diag(Stl^-1) (l)
 
   // interactions
-  const double* E;       // [n_env][lde]
+  const double* E;       // [n_env][lde], lde = n_pad, zero rows beyond n
   long long lde;
 
   // traits
@@ -74,6 +75,7 @@ struct DevScan {
 
   // genotypes
   const void* Gm; int gdt; long long ldg;
+  int g_vec;             // Gm 16-byte aligned and ldg % 4 == 0
   const int* col_src; const int* col_env;   // [m*r] for the whole loaded block
 
   // chunk

@@ -20,6 +20,10 @@
 // workgroup ids to the 8 XCDs round-robin, and each XCD receives a
 // contiguous range of tiles (row-tile major), so the tiles that share an A
 // row block run on the same XCD and reuse it from that XCD's L2.
+//
+// launch_gemm times these kernels and the library GEMMs of
+// oagpu_gemm_lib.hip on the first operands of each shape class and keeps the
+// fastest.
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -27,6 +31,7 @@
 #include <tuple>
 
 #include "oagpu_kernels.h"
+#include "oagpu_gemm_lib.h"
 
 namespace oa {
 
@@ -40,7 +45,7 @@ __device__ __forceinline__ int xcd_tile(int id, int total) {
 // supplies k = 4g + step.
 __global__ __launch_bounds__(256, 2) void k_gemm_f64(const double* __restrict__ A,
                                                     const double* __restrict__ B,
-                                                    double* __restrict__ C, int M, int N, int K,
+                                                    double* __restrict__ C, int M, int N, int K, int lda, int ldb,
                                                     int ldc, int tiles_n, int total) {
   const int tile = xcd_tile(blockIdx.x, total);
   const int tm = tile / tiles_n, tn = tile - tm * tiles_n;
@@ -57,8 +62,8 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f64(const double* __restrict__ 
   const double* Bp[4];
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
-    Ap[a] = A + (size_t)(row0 + 16 * a + i) * K + 4 * g;
-    Bp[a] = B + (size_t)(col0 + 16 * a + i) * K + 4 * g;
+    Ap[a] = A + (size_t)(row0 + 16 * a + i) * lda + 4 * g;
+    Bp[a] = B + (size_t)(col0 + 16 * a + i) * ldb + 4 * g;
   }
   for (int k0 = 0; k0 < K; k0 += 16) {
     d4 av[4], bv[4];
@@ -92,7 +97,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f64(const double* __restrict__ 
 // FP32: v_mfma_f32_16x16x4f32, same structure with 4-float runs.
 __global__ __launch_bounds__(256, 2) void k_gemm_f32(const float* __restrict__ A,
                                                     const float* __restrict__ B,
-                                                    float* __restrict__ C, int M, int N, int K,
+                                                    float* __restrict__ C, int M, int N, int K, int lda, int ldb,
                                                     int ldc, int tiles_n, int total) {
   const int tile = xcd_tile(blockIdx.x, total);
   const int tm = tile / tiles_n, tn = tile - tm * tiles_n;
@@ -109,8 +114,8 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f32(const float* __restrict__ A
   const float* Bp[4];
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
-    Ap[a] = A + (size_t)(row0 + 16 * a + i) * K + 4 * g;
-    Bp[a] = B + (size_t)(col0 + 16 * a + i) * K + 4 * g;
+    Ap[a] = A + (size_t)(row0 + 16 * a + i) * lda + 4 * g;
+    Bp[a] = B + (size_t)(col0 + 16 * a + i) * ldb + 4 * g;
   }
   for (int k0 = 0; k0 < K; k0 += 16) {
     f4 av[4], bv[4];
@@ -150,7 +155,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf16x3(const uint16_t* __restri
                                                        const uint16_t* __restrict__ Al,
                                                        const uint16_t* __restrict__ Bh,
                                                        const uint16_t* __restrict__ Bl,
-                                                       float* __restrict__ C, int M, int N, int K,
+                                                       float* __restrict__ C, int M, int N, int K, int lda, int ldb,
                                                        int ldc, int tiles_n, int total) {
   const int tile = xcd_tile(blockIdx.x, total);
   const int tm = tile / tiles_n, tn = tile - tm * tiles_n;
@@ -166,8 +171,8 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf16x3(const uint16_t* __restri
   size_t ao[4], bo[4];
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
-    ao[a] = (size_t)(row0 + 16 * a + i) * K + 8 * g;
-    bo[a] = (size_t)(col0 + 16 * a + i) * K + 8 * g;
+    ao[a] = (size_t)(row0 + 16 * a + i) * lda + 8 * g;
+    bo[a] = (size_t)(col0 + 16 * a + i) * ldb + 8 * g;
   }
   for (int k0 = 0; k0 < K; k0 += 32) {
     u4 ah[4], al[4], bh[4], bl[4];
@@ -211,7 +216,7 @@ __global__ __launch_bounds__(256, 1) void k_gemm_bf16x3_lds(const uint16_t* __re
                                                            const uint16_t* __restrict__ Al,
                                                            const uint16_t* __restrict__ Bh,
                                                            const uint16_t* __restrict__ Bl,
-                                                           float* __restrict__ C, int M, int N, int K,
+                                                           float* __restrict__ C, int M, int N, int K, int lda, int ldb,
                                                            int ldc, int tiles_n, int total) {
   __shared__ __attribute__((aligned(16))) uint16_t sm[2][4][128 * LDS_ROW];   // [stage][Ah,Al,Bh,Bl]
   const int tile = xcd_tile(blockIdx.x, total);
@@ -227,8 +232,8 @@ __global__ __launch_bounds__(256, 1) void k_gemm_bf16x3_lds(const uint16_t* __re
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
     const int c = tid + q * 256, row = c >> 2, ch = c & 3;
-    goff[q][0] = (size_t)(tm * 128 + row) * K + 8 * ch;
-    goff[q][1] = (size_t)(tn * 128 + row) * K + 8 * ch;
+    goff[q][0] = (size_t)(tm * 128 + row) * lda + 8 * ch;
+    goff[q][1] = (size_t)(tn * 128 + row) * ldb + 8 * ch;
     soff[q] = row * LDS_ROW + 8 * ch;
   }
   u4 st[2][4];
@@ -306,7 +311,7 @@ __global__ __launch_bounds__(256, 1) void k_gemm_bf16x3_big(const uint16_t* __re
                                                            const uint16_t* __restrict__ Al,
                                                            const uint16_t* __restrict__ Bh,
                                                            const uint16_t* __restrict__ Bl,
-                                                           float* __restrict__ C, int M, int N, int K,
+                                                           float* __restrict__ C, int M, int N, int K, int lda, int ldb,
                                                            int ldc, int tiles_n, int total) {
   constexpr int BM = 32 * WM_BLK;          // rows of A per workgroup (2 waves x 16*WM_BLK)
   constexpr int BN = 128;
@@ -326,13 +331,13 @@ __global__ __launch_bounds__(256, 1) void k_gemm_bf16x3_big(const uint16_t* __re
 #pragma unroll
   for (int q = 0; q < A_CH; ++q) {
     const int c = tid + q * 256, row = c >> 2, ch = c & 3;
-    ga[q] = (size_t)(tm * BM + row) * K + 8 * ch;
+    ga[q] = (size_t)(tm * BM + row) * lda + 8 * ch;
     sa_off[q] = row * LDS_ROW + 8 * ch;
   }
 #pragma unroll
   for (int q = 0; q < B_CH; ++q) {
     const int c = tid + q * 256, row = c >> 2, ch = c & 3;
-    gb[q] = (size_t)(tn * BN + row) * K + 8 * ch;
+    gb[q] = (size_t)(tn * BN + row) * ldb + 8 * ch;
     sb_off[q] = row * LDS_ROW + 8 * ch;
   }
   u4 sta[A_CH][2], stb[B_CH][2];
@@ -421,7 +426,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf16x3_pf2(const uint16_t* __re
                                                            const uint16_t* __restrict__ Al,
                                                            const uint16_t* __restrict__ Bh,
                                                            const uint16_t* __restrict__ Bl,
-                                                           float* __restrict__ C, int M, int N, int K,
+                                                           float* __restrict__ C, int M, int N, int K, int lda, int ldb,
                                                            int ldc, int tiles_n, int total) {
   __shared__ __attribute__((aligned(16))) uint16_t sm[2][4][128 * LDS_ROW];   // [stage][Ah,Al,Bh,Bl]
   const int tile = xcd_tile(blockIdx.x, total);
@@ -435,8 +440,8 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf16x3_pf2(const uint16_t* __re
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
     const int c = tid + q * 256, row = c >> 2, ch = c & 3;
-    goff[q][0] = (size_t)(tm * 128 + row) * K + 8 * ch;
-    goff[q][1] = (size_t)(tn * 128 + row) * K + 8 * ch;
+    goff[q][0] = (size_t)(tm * 128 + row) * lda + 8 * ch;
+    goff[q][1] = (size_t)(tn * 128 + row) * ldb + 8 * ch;
     soff[q] = row * LDS_ROW + 8 * ch;
   }
   u4 ra[2][4], rb[2][4];   // two register stages
@@ -530,52 +535,70 @@ struct GemmVariant {
   const char* name;
   int prec;
   int bm, bn;   // workgroup tile
-  void (*launch)(const void*, const void*, const void*, const void*, void*, int, int, int, int, hipStream_t);
+  // false: the variant cannot run this shape and launched nothing
+  bool (*launch)(const void*, const void*, const void*, const void*, void*, int, int, int, int, int, int,
+                 hipStream_t);
 };
 
-static void v_f64(const void* Ah, const void*, const void* Bh, const void*, void* C, int M, int N, int K,
-                  int ldc, hipStream_t st) {
+static bool v_f64(const void* Ah, const void*, const void* Bh, const void*, void* C, int M, int N, int K,
+                  int lda, int ldb, int ldc, hipStream_t st) {
   const int tiles_m = (M + 127) / 128, tiles_n = (N + 127) / 128, total = tiles_m * tiles_n;
   hipLaunchKernelGGL(k_gemm_f64, dim3(total), dim3(256), 0, st, (const double*)Ah, (const double*)Bh,
-                     (double*)C, M, N, K, ldc, tiles_n, total);
+                     (double*)C, M, N, K, lda, ldb, ldc, tiles_n, total);
+  return true;
 }
-static void v_f32(const void* Ah, const void*, const void* Bh, const void*, void* C, int M, int N, int K,
-                  int ldc, hipStream_t st) {
+static bool v_f32(const void* Ah, const void*, const void* Bh, const void*, void* C, int M, int N, int K,
+                  int lda, int ldb, int ldc, hipStream_t st) {
   const int tiles_m = (M + 127) / 128, tiles_n = (N + 127) / 128, total = tiles_m * tiles_n;
   hipLaunchKernelGGL(k_gemm_f32, dim3(total), dim3(256), 0, st, (const float*)Ah, (const float*)Bh,
-                     (float*)C, M, N, K, ldc, tiles_n, total);
+                     (float*)C, M, N, K, lda, ldb, ldc, tiles_n, total);
+  return true;
 }
-static void v_bf_direct(const void* Ah, const void* Al, const void* Bh, const void* Bl, void* C, int M, int N,
-                        int K, int ldc, hipStream_t st) {
+static bool v_bf_direct(const void* Ah, const void* Al, const void* Bh, const void* Bl, void* C, int M, int N,
+                        int K, int lda, int ldb, int ldc, hipStream_t st) {
   const int tiles_m = (M + 127) / 128, tiles_n = (N + 127) / 128, total = tiles_m * tiles_n;
   hipLaunchKernelGGL(k_gemm_bf16x3, dim3(total), dim3(256), 0, st, (const uint16_t*)Ah, (const uint16_t*)Al,
-                     (const uint16_t*)Bh, (const uint16_t*)Bl, (float*)C, M, N, K, ldc, tiles_n, total);
+                     (const uint16_t*)Bh, (const uint16_t*)Bl, (float*)C, M, N, K, lda, ldb, ldc, tiles_n, total);
+  return true;
 }
-static void v_bf_lds(const void* Ah, const void* Al, const void* Bh, const void* Bl, void* C, int M, int N,
-                     int K, int ldc, hipStream_t st) {
+static bool v_bf_lds(const void* Ah, const void* Al, const void* Bh, const void* Bl, void* C, int M, int N,
+                     int K, int lda, int ldb, int ldc, hipStream_t st) {
   const int tiles_m = (M + 127) / 128, tiles_n = (N + 127) / 128, total = tiles_m * tiles_n;
   hipLaunchKernelGGL(k_gemm_bf16x3_lds, dim3(total), dim3(256), 0, st, (const uint16_t*)Ah,
-                     (const uint16_t*)Al, (const uint16_t*)Bh, (const uint16_t*)Bl, (float*)C, M, N, K, ldc,
+                     (const uint16_t*)Al, (const uint16_t*)Bh, (const uint16_t*)Bl, (float*)C, M, N, K, lda, ldb, ldc,
                      tiles_n, total);
+  return true;
 }
-static void v_bf_pf2(const void* Ah, const void* Al, const void* Bh, const void* Bl, void* C, int M, int N,
-                     int K, int ldc, hipStream_t st) {
+static bool v_bf_pf2(const void* Ah, const void* Al, const void* Bh, const void* Bl, void* C, int M, int N,
+                     int K, int lda, int ldb, int ldc, hipStream_t st) {
   const int tiles_m = (M + 127) / 128, tiles_n = (N + 127) / 128, total = tiles_m * tiles_n;
   hipLaunchKernelGGL(k_gemm_bf16x3_pf2, dim3(total), dim3(256), 0, st, (const uint16_t*)Ah,
-                     (const uint16_t*)Al, (const uint16_t*)Bh, (const uint16_t*)Bl, (float*)C, M, N, K, ldc,
+                     (const uint16_t*)Al, (const uint16_t*)Bh, (const uint16_t*)Bl, (float*)C, M, N, K, lda, ldb, ldc,
                      tiles_n, total);
+  return true;
 }
-static void v_bf_big(const void* Ah, const void* Al, const void* Bh, const void* Bl, void* C, int M, int N,
-                     int K, int ldc, hipStream_t st) {
+static bool v_bf_big(const void* Ah, const void* Al, const void* Bh, const void* Bl, void* C, int M, int N,
+                     int K, int lda, int ldb, int ldc, hipStream_t st) {
   const int tiles_m = (M + 255) / 256, tiles_n = (N + 127) / 128, total = tiles_m * tiles_n;
   hipLaunchKernelGGL(k_gemm_bf16x3_big<8>, dim3(total), dim3(256), 0, st, (const uint16_t*)Ah,
-                     (const uint16_t*)Al, (const uint16_t*)Bh, (const uint16_t*)Bl, (float*)C, M, N, K, ldc,
+                     (const uint16_t*)Al, (const uint16_t*)Bh, (const uint16_t*)Bl, (float*)C, M, N, K, lda, ldb, ldc,
                      tiles_n, total);
+  return true;
 }
 
+static bool v_lib_bf16x3(const void* Ah, const void* Al, const void* Bh, const void* Bl, void* C, int M, int N,
+                         int K, int lda, int ldb, int ldc, hipStream_t st) {
+  return lib_gemm_bf16x3(Ah, Al, Bh, Bl, C, M, N, K, lda, ldb, ldc, st);
+}
+static bool v_lib_f32(const void* Ah, const void*, const void* Bh, const void*, void* C, int M, int N, int K,
+                      int lda, int ldb, int ldc, hipStream_t st) {
+  return lib_gemm_f32(Ah, Bh, C, M, N, K, lda, ldb, ldc, st);
+}
 static const GemmVariant kVariants[] = {
     {"f64_direct_128x128", PREC_FP64, 128, 128, v_f64},
     {"f32_direct_128x128", PREC_FP32, 128, 128, v_f32},
+    {"f32_hipblaslt", PREC_FP32, 0, 0, v_lib_f32},
+    {"bf16x3_hipblaslt", PREC_BF16X3, 0, 0, v_lib_bf16x3},
     {"bf16x3_lds_pf2_128x128", PREC_BF16X3, 128, 128, v_bf_pf2},
     {"bf16x3_lds_256x128", PREC_BF16X3, 256, 128, v_bf_big},
     {"bf16x3_lds_128x128", PREC_BF16X3, 128, 128, v_bf_lds},
@@ -610,7 +633,7 @@ const char* gemm_variant_name(int prec, int M, int N, int K) {
 int gemm_row_pad() { return 256; }
 
 void launch_gemm(int prec, const void* Ahi, const void* Alo, const void* Bhi, const void* Blo,
-                 void* C, int M, int N, int K, int ldc, hipStream_t st) {
+                 void* C, int M, int N, int K, int lda, int ldb, int ldc, hipStream_t st) {
   int v = pinned_variant(prec);
   if (v < 0) {
     const int mclass = M >= 65536 ? 2 : (M >= 4096 ? 1 : 0);
@@ -628,21 +651,22 @@ void launch_gemm(int prec, const void* Ahi, const void* Alo, const void* Bhi, co
       }
     if (v < 0 && nvar == 1) v = only;
     if (v < 0) {
-      // time every variant of this precision on these operands
+      // time every variant of this precision on these operands; one that
+      // cannot run the shape drops out
       float best = 1e30f;
       hipEvent_t e0, e1;
       (void)hipEventCreate(&e0);
       (void)hipEventCreate(&e1);
       for (int c = 0; c < kNumVariants; ++c) {
         if (kVariants[c].prec != prec) continue;
-        kVariants[c].launch(Ahi, Alo, Bhi, Blo, C, M, N, K, ldc, st);   // warm
+        if (!kVariants[c].launch(Ahi, Alo, Bhi, Blo, C, M, N, K, lda, ldb, ldc, st)) continue;   // warm
         (void)hipEventRecord(e0, st);
-        kVariants[c].launch(Ahi, Alo, Bhi, Blo, C, M, N, K, ldc, st);
+        const bool ran = kVariants[c].launch(Ahi, Alo, Bhi, Blo, C, M, N, K, lda, ldb, ldc, st);
         (void)hipEventRecord(e1, st);
         (void)hipEventSynchronize(e1);
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e0, e1);
-        if (ms < best) {
+        if (ran && ms < best) {
           best = ms;
           v = c;
         }
@@ -653,7 +677,15 @@ void launch_gemm(int prec, const void* Ahi, const void* Alo, const void* Bhi, co
       g_tuned[key] = v;
     }
   }
-  kVariants[v].launch(Ahi, Alo, Bhi, Blo, C, M, N, K, ldc, st);
+  if (v >= 0 && kVariants[v].launch(Ahi, Alo, Bhi, Blo, C, M, N, K, lda, ldb, ldc, st)) return;
+  // a pinned library variant that cannot run the shape: the first
+  // hand-written kernel of this precision
+  for (int c = 0; c < kNumVariants; ++c)
+    if (kVariants[c].prec == prec && kVariants[c].bm > 0) {
+      kVariants[c].launch(Ahi, Alo, Bhi, Blo, C, M, N, K, lda, ldb, ldc, st);
+      return;
+    }
 }
+
 
 }  // namespace oa

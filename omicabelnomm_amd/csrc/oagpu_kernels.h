@@ -32,10 +32,12 @@ void launch_fallback(const DevScan& d, int lb, hipStream_t st);
 void launch_reset_counter(unsigned long long* c, int idx, hipStream_t st);
 
 // contraction (oagpu_gemm.hip): C[c][j] = sum_k A[c][k] B[j][k]
-// A: M rows of K (padded to gemm_row_pad() rows), B: N rows of K (padded
-// to 128 rows), K a multiple of 64.  The variant is chosen by timing.
+// A: M rows of K, row stride lda (padded to gemm_row_pad() rows), B: N rows
+// of K, row stride ldb (padded to 128 rows), K a multiple of 64.  The
+// split-BF16 operands are interleaved per row: A rows [hi | lo], B rows
+// [lo | hi], strides 2K.  The variant is chosen by timing.
 void launch_gemm(int prec, const void* Ahi, const void* Alo, const void* Bhi, const void* Blo,
-                 void* C, int M, int N, int K, int ldc, hipStream_t st);
+                 void* C, int M, int N, int K, int lda, int ldb, int ldc, hipStream_t st);
 const char* gemm_variant_name(int prec, int M, int N, int K);
 int gemm_row_pad();
 

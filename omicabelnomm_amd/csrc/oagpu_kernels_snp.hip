@@ -34,9 +34,28 @@ __device__ __forceinline__ double geno_value(const DevScan& d, int src, int env,
   return v;
 }
 
+// Output column (src, env) at rows k0..k0+3 (k0 a multiple of 4), NaN if
+// missing, 0 beyond n.  Interior runs are one vector load.
+__device__ __forceinline__ void geno_value4(const DevScan& d, int src, int env, int k0, double* x) {
+  const size_t base = (size_t)src * d.ldg + k0;
+  if (k0 + 4 <= d.n) {
+    load_elem4(d.Gm, d.gdt, d.g_vec != 0, base, x);
+  } else {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) x[e] = (k0 + e < d.n) ? load_elem(d.Gm, d.gdt, base + e) : 0.0;
+  }
+  if (env >= 0) {   // E rows are padded to n_pad, so the run is always in bounds
+    const double* ep = d.E + (size_t)env * d.lde + k0;
+    const double2 a = *reinterpret_cast<const double2*>(ep);
+    const double2 b = *reinterpret_cast<const double2*>(ep + 2);
+    x[0] *= a.x; x[1] *= a.y; x[2] *= b.x; x[3] *= b.y;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Operand preparation, prep_spw<R>() SNPs per block: mean imputation, interaction
-// columns, the contraction operand A (column s*R+c, rows n..n_pad-1 zero),
+// columns, the contraction operand A (row s*R+c at stride ldop, entries
+// n..n_pad-1 zero),
 // Sbr = XR_s'XR_s and -- when the patterns' missing rows are few -- the
 // imputed values at those rows (u-major, so the corrections kernel reads
 // them coalesced).  Each thread handles runs of 4 consecutive rows, so loads
@@ -85,13 +104,19 @@ __global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo) {
   double sum[NV], cnt[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) { sum[v] = 0.0; cnt[v] = 0.0; }
-  for (int k = tid; k < d.n; k += 256) {
+  // two 4-row runs in flight per thread: the loop is latency bound
+#pragma unroll 2
+  for (int k0 = 4 * tid; k0 < d.n; k0 += 1024) {
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-      double x = load_elem(d.Gm, d.gdt, (size_t)src[v] * d.ldg + k);
-      if (env[v] >= 0) x *= d.E[(size_t)env[v] * d.lde + k];
-      if (x == x) sum[v] += x;
-      else cnt[v] += 1.0;
+      double x[4];
+      geno_value4(d, src[v], env[v], k0, x);
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (k0 + e < d.n) {
+          if (x[e] == x[e]) sum[v] += x[e];
+          else cnt[v] += 1.0;
+        }
     }
   }
   double mean[NV];
@@ -105,21 +130,16 @@ __global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo) {
   double acc[PREP_SPW * RR];
 #pragma unroll
   for (int e = 0; e < PREP_SPW * RR; ++e) acc[e] = 0.0;
+#pragma unroll 2
   for (int k0 = 4 * tid; k0 < d.n_pad; k0 += 1024) {
     double x[NV][4];
 #pragma unroll
-    for (int v = 0; v < NV; ++v)
+    for (int v = 0; v < NV; ++v) {
+      geno_value4(d, src[v], env[v], k0, x[v]);
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int k = k0 + e;
-        double val = 0.0;
-        if (k < d.n) {
-          val = load_elem(d.Gm, d.gdt, (size_t)src[v] * d.ldg + k);
-          if (env[v] >= 0) val *= d.E[(size_t)env[v] * d.lde + k];
-          if (val != val) val = mean[v];
-        }
-        x[v][e] = val;
-      }
+      for (int e = 0; e < 4; ++e)
+        x[v][e] = (k0 + e < d.n) ? ((x[v][e] == x[v][e]) ? x[v][e] : mean[v]) : 0.0;
+    }
     if (CORR) {
       // rows of U: their imputed values, u-major, for the corrections
       const int4 ui = *reinterpret_cast<const int4*>(d.uidx + k0);
@@ -138,7 +158,7 @@ __global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo) {
       if (!live[q]) continue;
 #pragma unroll
       for (int c = 0; c < R; ++c)
-        store_op4<OP>(Ahi, Alo, (size_t)((sb + q) * R + c) * d.n_pad + k0, x[q * R + c]);
+        store_op4<OP>(Ahi, Alo, (size_t)((sb + q) * R + c) * d.ldop + k0, x[q * R + c]);
 #pragma unroll
       for (int e = 0; e < 4; ++e)
 #pragma unroll
@@ -443,11 +463,18 @@ __device__ void stats_emit(const DevScan& d, int y, int s, const double* b, doub
   double sig = 0.0;
   if (rss > 0.0 && (R2 > d.min_r2_store || d.store_all)) {
     sig = sqrt(rss / (double)n_c);
+    const double s2 = rss / (double)n_c * vf;
+    const double tt = d.t_star * d.t_star;   // exact: t_star is a float
 #pragma unroll
     for (int h = P - 1; h >= 0; --h) {
       if (h < p && h >= d.start_h) {
         const double vxh = vx[h];
         if (vxh < 0.0) atomicOr(&d.counters[CNT_ERR], 1ull);
+        // |b/se| > t_star  <=>  b^2 > t_star^2 se^2 (se^2 >= 0): the
+        // square roots and the division are only needed for what is kept
+        const double se2 = s2 * vxh;
+        const bool pass = dense || d.store_all || (se2 >= 0.0 && b[h] * b[h] > tt * se2);
+        if (!pass) continue;
         const double se = sig * sqrt(vxh * vf);
         const double tv = fabs(b[h] / se);
         if (dense) {
@@ -902,13 +929,21 @@ __global__ __launch_bounds__(256) void k_patsnp(DevScan d) {
   for (int b2 = 0; b2 < R; ++b2)
 #pragma unroll
     for (int a = 0; a <= b2; ++a) {
-      out[T::US + upk(a, b2)] = Sc[upk(a, b2)];
+      // diagonal stored as its reciprocal: the pair solves multiply
+      out[T::US + upk(a, b2)] = (a == b2) ? 1.0 / Sc[upk(a, b2)] : Sc[upk(a, b2)];
       out[T::H + upk(a, b2)] = (a >= l) ? (a == b2 ? 1.0 : 2.0) * Crr[upk(a, b2)] : 0.0;
     }
   out[T::OK] = ok ? 1.0 : 0.0;
 }
 
-// One thread per (trait, SNP) pair, reading the (SNP, pattern) table.
+// One thread per trait and EPI_SPT consecutive SNPs, reading the (SNP,
+// pattern) table.  A wave covers 64 traits, so each SNP's table row is
+// wave-uniform when the traits share a pattern.  (Several SNPs per thread
+// amortise the trait's loads but raise the register count from 32 to over
+// 130 and measured slower: 15 ms at 2 and 17.7 ms at 4 against 11.1 ms at 1
+// for 200,000 SNPs x 2,000 traits.)
+constexpr int EPI_SPT = 1;
+
 template <int R, int LB, typename CT>
 __global__ __launch_bounds__(256) void k_epilogue_pst(DevScan d) {
   using T = PatSnp<R, LB>;
@@ -916,85 +951,97 @@ __global__ __launch_bounds__(256) void k_epilogue_pst(DevScan d) {
   constexpr int P = LB + R;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int s = blockIdx.y * 4 + w;
-  if (s >= d.mc) return;
+  const int s0 = (blockIdx.y * 4 + w) * EPI_SPT;
+  if (s0 >= d.mc) return;
   const int y = blockIdx.x * 64 + lane;
   if (y >= d.t) return;
   const int l = d.l;
   const int g = d.ypat[y];
-  const double* tb = d.pst + ((size_t)s * d.G + g) * T::W;
+  const bool quirk = (R > 1) && d.pat_cnt[g] > 0;
+  const double rss0 = d.rss0[y];
+  const bool bl_always = d.need_bl || quirk || d.yzero[y] > 0;
+  const double guard = RSS_GUARD * d.yy[y];
+  double zt[LB], ay[LB];
+#pragma unroll
+  for (int j = 0; j < LB; ++j) {
+    zt[j] = (j < l) ? d.zt[(size_t)j * d.ldt + y] : 0.0;
+    ay[j] = (j < l) ? d.ay[(size_t)j * d.ldt + y] : 0.0;
+  }
   const CT* C = reinterpret_cast<const CT*>(d.Cbuf);
-  double rb[R];
+#pragma unroll 1
+  for (int q = 0; q < EPI_SPT; ++q) {
+    const int s = s0 + q;
+    if (s >= d.mc) break;
+    const double* tb = d.pst + ((size_t)s * d.G + g) * T::W;
+    double rb[R];
 #pragma unroll
-  for (int c = 0; c < R; ++c) rb[c] = (double)C[(size_t)(s * R + c) * d.ldc + y];
-  const bool ok = tb[T::OK] != 0.0;
-  double b[P];
+    for (int c = 0; c < R; ++c) rb[c] = (double)C[(size_t)(s * R + c) * d.ldc + y];
+    const bool ok = tb[T::OK] != 0.0;
+    double b[P];
 #pragma unroll
-  for (int h = 0; h < P; ++h) b[h] = 0.0;
-  double rss = 0.0;
-  if (ok) {
-    double z[R];
+    for (int h = 0; h < P; ++h) b[h] = 0.0;
+    double rss = 0.0;
+    if (ok) {
+      double z[R];
 #pragma unroll
-    for (int c = 0; c < R; ++c) z[c] = rb[c];
-#pragma unroll
-    for (int j = 0; j < LB; ++j)
-      if (j < l) {
-        const double ztj = d.zt[(size_t)j * d.ldt + y];
-#pragma unroll
-        for (int c = 0; c < R; ++c) z[c] -= tb[T::X + j * R + c] * ztj;
-      }
-    double us[RR];
-#pragma unroll
-    for (int e = 0; e < RR; ++e) us[e] = tb[T::US + e];
-#pragma unroll
-    for (int i = 0; i < R; ++i) {   // U_S' z = num
-      double v = z[i];
-#pragma unroll
-      for (int k = 0; k < i; ++k) v -= us[upk(k, i)] * z[k];
-      z[i] = v / us[upk(i, i)];
-    }
-    double zz = 0.0;
-#pragma unroll
-    for (int c = 0; c < R; ++c) zz += z[c] * z[c];
-    double br[R];
-#pragma unroll
-    for (int i = R - 1; i >= 0; --i) {   // U_S b_r = z
-      double v = z[i];
-#pragma unroll
-      for (int k = i + 1; k < R; ++k) v -= us[upk(i, k)] * br[k];
-      br[i] = v / us[upk(i, i)];
-    }
-    rss = d.rss0[y] - zz;
-#pragma unroll
-    for (int c = 0; c < R; ++c) b[LB + c] = br[c];
-    const bool quirk = (R > 1) && d.pat_cnt[g] > 0;
-    const bool need_bl = d.need_bl || quirk || d.yzero[y] > 0 || !(rss >= RSS_GUARD * d.yy[y]);
-    if (need_bl) {
-      double q = 0.0;
+      for (int c = 0; c < R; ++c) z[c] = rb[c];
 #pragma unroll
       for (int j = 0; j < LB; ++j)
         if (j < l) {
-          double v = d.ay[(size_t)j * d.ldt + y];
 #pragma unroll
-          for (int c = 0; c < R; ++c) v -= tb[T::M + j * R + c] * br[c];
-          b[j] = v;
-          if (quirk) {
-            double kv = 0.0;
-#pragma unroll
-            for (int c = 0; c < R; ++c) kv += tb[T::K + j * R + c] * br[c];
-            q += 2.0 * v * kv;
-          }
+          for (int c = 0; c < R; ++c) z[c] -= tb[T::X + j * R + c] * zt[j];
         }
-      if (quirk) {
+      double us[RR];
 #pragma unroll
-        for (int b2 = 0; b2 < R; ++b2)
+      for (int e = 0; e < RR; ++e) us[e] = tb[T::US + e];
 #pragma unroll
-          for (int a = 0; a <= b2; ++a) q -= tb[T::H + upk(a, b2)] * br[a] * br[b2];
-        rss += q;
+      for (int i = 0; i < R; ++i) {   // U_S' z = num (diagonal reciprocals in the table)
+        double v = z[i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) v -= us[upk(k, i)] * z[k];
+        z[i] = v * us[upk(i, i)];
+      }
+      double zz = 0.0;
+#pragma unroll
+      for (int c = 0; c < R; ++c) zz += z[c] * z[c];
+      double br[R];
+#pragma unroll
+      for (int i = R - 1; i >= 0; --i) {   // U_S b_r = z
+        double v = z[i];
+#pragma unroll
+        for (int k = i + 1; k < R; ++k) v -= us[upk(i, k)] * br[k];
+        br[i] = v * us[upk(i, i)];
+      }
+      rss = rss0 - zz;
+#pragma unroll
+      for (int c = 0; c < R; ++c) b[LB + c] = br[c];
+      if (bl_always || !(rss >= guard)) {
+        double qv = 0.0;
+#pragma unroll
+        for (int j = 0; j < LB; ++j)
+          if (j < l) {
+            double v = ay[j];
+#pragma unroll
+            for (int c = 0; c < R; ++c) v -= tb[T::M + j * R + c] * br[c];
+            b[j] = v;
+            if (quirk) {
+              double kv = 0.0;
+#pragma unroll
+              for (int c = 0; c < R; ++c) kv += tb[T::K + j * R + c] * br[c];
+              qv += 2.0 * v * kv;
+            }
+          }
+        if (quirk) {
+#pragma unroll
+          for (int b2 = 0; b2 < R; ++b2)
+#pragma unroll
+            for (int a = 0; a <= b2; ++a) qv -= tb[T::H + upk(a, b2)] * br[a] * br[b2];
+          rss += qv;
+        }
       }
     }
+    finish_pair<R, LB>(d, y, s, ok, b, rb, rss);
   }
-  finish_pair<R, LB>(d, y, s, ok, b, rb, rss);
 }
 
 // Explicit residual pass for the pairs the epilogue handed over: one wave
@@ -1119,7 +1166,7 @@ void launch_patsnp(const DevScan& d, int lb, hipStream_t st) {
 int patsnp_width(int r, int lb) { return 3 * lb * r + r * (r + 1) + 1; }
 
 void launch_epilogue_pst(const DevScan& d, int prec, int lb, hipStream_t st) {
-  dim3 grid((d.t + 63) / 64, (d.mc + 3) / 4), block(256);
+  dim3 grid((d.t + 63) / 64, (d.mc + 4 * EPI_SPT - 1) / (4 * EPI_SPT)), block(256);
 #define OA_EQ(R_, LB_)                                                                              \
   if (prec == PREC_FP64) hipLaunchKernelGGL((k_epilogue_pst<R_, LB_, double>), grid, block, 0, st, d); \
   else hipLaunchKernelGGL((k_epilogue_pst<R_, LB_, float>), grid, block, 0, st, d);

@@ -21,7 +21,7 @@ __global__ __launch_bounds__(256) void k_ystats(DevScan d, void* Bhi, void* Blo,
   const int y = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const size_t ybase = (size_t)y * d.ldy;
-  const size_t bbase = (size_t)y * d.n_pad;
+  const size_t bbase = (size_t)y * d.ldop;
   __shared__ int wcnt[4];
   __shared__ double shd[4];
   __shared__ long long shl[4];

@@ -113,3 +113,28 @@ def test_mixed_precision_bounds(precision, oracle_mod):
     hit_g = (d["pval"] < 1e-8) & d["kept"].astype(bool)
     np.testing.assert_array_equal(hit_g, hit_o)
     assert hit_o.sum() > 0
+
+
+GEMM_VARIANTS = {
+    "fp64": ["f64_direct_128x128"],
+    "fp32": ["f32_direct_128x128", "f32_hipblaslt"],
+    "bf16x3": ["bf16x3_hipblaslt", "bf16x3_lds_pf2_128x128", "bf16x3_lds_256x128", "bf16x3_lds_128x128",
+               "bf16x3_direct_128x128"],
+}
+
+
+@pytest.mark.parametrize("precision,variant", [(p, v) for p, vs in GEMM_VARIANTS.items() for v in vs])
+def test_gemm_variant(precision, variant, oracle_mod, monkeypatch):
+    """Every contraction kernel the autotuner may pick, pinned by name."""
+    monkeypatch.setenv("OAGPU_GEMM_VARIANT", variant)
+    data = synth.make(n=700, l=4, m=70, t=150, interaction=True, miss="groups", pattern_groups=3,
+                      pattern_rate=0.03, planted=12, beta=0.3, seed=33)
+    print("case gemm", precision, variant, flush=True)
+    g = run_gpu(data, precision, min_r2_store=-10)
+    o = run_oracle(oracle_mod, data, min_r2_store=-10)
+    if precision == "fp64":
+        compare_fp64(g, o)
+    else:
+        db, dl = precision_report(g, o)
+        print(variant, "max|db|/SE", db, "max|d-log10p|", dl, flush=True)
+        assert db < 1e-3 and dl < 1e-3
