@@ -87,6 +87,21 @@ def timed(sc, data, steps, warmup):
     return dt, res, gemm_ms / max(chunks, 1), chunks / steps
 
 
+def measured_traffic(variant, config):
+    """HBM bytes per contraction from the committed rocprofv3 counter
+    summary of this kernel and workload, or None."""
+    path = os.path.join(ROOT, "profiles", "c%d_contraction_traffic.json" % config)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    if rec.get("kernel") != variant:
+        return None
+    per_launch = 1024.0 * (rec["fetch_kib_per_launch"] * rec["fetch_scale"] +
+                           rec["write_kib_per_launch"] * rec["write_scale"])
+    return per_launch * rec["launches_per_contraction"]
+
+
 def cpu_baseline(data, budget_s):
     """FP64 CPU restatement (oracle) on the first m' SNPs x all traits."""
     from oracle import oracle as O
@@ -193,9 +208,11 @@ def main():
             "precision": args.precision, "chunks_per_step": chunks,
             "gen_seconds": round(gen_s, 2)},
         "roofline": {"bound": "mfma", "achieved": gemm_tflops, "peak": peak, "unit": "TFLOP/s",
-                     "frac": gemm_tflops / peak, "traffic": None,
+                     "frac": gemm_tflops / peak, "traffic": None if over else measured_traffic(variant, args.config),
                      "kernel": variant,
-                     "note": "algorithmic 2*n*r*m*(t+l) flops / HIP-event time of the contraction"},
+                     "note": "algorithmic 2*n*r*m*(t+l) flops / HIP-event time of the contraction; "
+                             "traffic = HBM bytes per contraction (rocprofv3 FETCH_SIZE, WRITE_SIZE, "
+                             "copy-calibrated) when a profile of this kernel is committed"},
         "cpu_baseline": cpu,
         "fp64": fp64,
         "speedup_vs_fp64": (value / fp64["value"]) if fp64 else None,

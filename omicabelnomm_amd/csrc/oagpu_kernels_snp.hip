@@ -100,73 +100,74 @@ __global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo) {
     const int gs = d.s0 + (live[q] ? sb + q : sb);
     col_desc<R>(d, gs, src + q * R, env + q * R);
   }
-  // pass 1: NaN counts and sums per column
-  double sum[NV], cnt[NV];
+  // One pass that assumes no missing genotype (the common case): missing
+  // values enter as 0 while the columns' sums and NaN counts accumulate.  A
+  // block whose columns do hold NaN repeats the pass with the means imputed,
+  // overwriting everything the first pass wrote.
+  double sum[NV], cnt[NV], mean[NV], acc[PREP_SPW * RR];
 #pragma unroll
-  for (int v = 0; v < NV; ++v) { sum[v] = 0.0; cnt[v] = 0.0; }
-  // two 4-row runs in flight per thread: the loop is latency bound
+  for (int v = 0; v < NV; ++v) { sum[v] = 0.0; cnt[v] = 0.0; mean[v] = 0.0; }
+  auto run = [&](bool impute) {
+#pragma unroll
+    for (int e = 0; e < PREP_SPW * RR; ++e) acc[e] = 0.0;
+    // two 4-row runs in flight per thread: the loop is latency bound
 #pragma unroll 2
-  for (int k0 = 4 * tid; k0 < d.n; k0 += 1024) {
+    for (int k0 = 4 * tid; k0 < d.n_pad; k0 += 1024) {
+      double x[NV][4];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      double x[4];
-      geno_value4(d, src[v], env[v], k0, x);
+      for (int v = 0; v < NV; ++v) {
+        geno_value4(d, src[v], env[v], k0, x[v]);
 #pragma unroll
-      for (int e = 0; e < 4; ++e)
-        if (k0 + e < d.n) {
-          if (x[e] == x[e]) sum[v] += x[e];
-          else cnt[v] += 1.0;
+        for (int e = 0; e < 4; ++e) {
+          const double xv = x[v][e];
+          if (k0 + e >= d.n) {
+            x[v][e] = 0.0;
+          } else if (xv == xv) {
+            if (!impute) sum[v] += xv;
+          } else {
+            if (!impute) cnt[v] += 1.0;
+            x[v][e] = mean[v];
+          }
         }
+      }
+      if (CORR) {
+        // rows of U: their imputed values, u-major, for the corrections
+        const int4 ui = *reinterpret_cast<const int4*>(d.uidx + k0);
+        const int uu[4] = {ui.x, ui.y, ui.z, ui.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (uu[e] >= 0) {
+            double* dst = d.Xu + (size_t)uu[e] * d.ldxu + (size_t)sb * R;
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+              if (live[v / R]) dst[v] = x[v][e];
+          }
+      }
+#pragma unroll
+      for (int q = 0; q < PREP_SPW; ++q) {
+        if (!live[q]) continue;
+#pragma unroll
+        for (int c = 0; c < R; ++c)
+          store_op4<OP>(Ahi, Alo, (size_t)((sb + q) * R + c) * d.ldop + k0, x[q * R + c]);
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+          for (int b = 0; b < R; ++b)
+#pragma unroll
+            for (int a = 0; a <= b; ++a) acc[q * RR + upk(a, b)] += x[q * R + a][e] * x[q * R + b][e];
+      }
     }
-  }
-  double mean[NV];
+  };
+  run(false);
+  double missing = 0.0;
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
     const double sc = block_sum256(sum[v], shd);
     const double nc = block_sum256(cnt[v], shd);
     mean[v] = sc / ((double)d.n - nc);
+    if (live[v / R]) missing += nc;
   }
-  // pass 2: operand, Sbr, corrections
-  double acc[PREP_SPW * RR];
-#pragma unroll
-  for (int e = 0; e < PREP_SPW * RR; ++e) acc[e] = 0.0;
-#pragma unroll 2
-  for (int k0 = 4 * tid; k0 < d.n_pad; k0 += 1024) {
-    double x[NV][4];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      geno_value4(d, src[v], env[v], k0, x[v]);
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-        x[v][e] = (k0 + e < d.n) ? ((x[v][e] == x[v][e]) ? x[v][e] : mean[v]) : 0.0;
-    }
-    if (CORR) {
-      // rows of U: their imputed values, u-major, for the corrections
-      const int4 ui = *reinterpret_cast<const int4*>(d.uidx + k0);
-      const int uu[4] = {ui.x, ui.y, ui.z, ui.w};
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-        if (uu[e] >= 0) {
-          double* dst = d.Xu + (size_t)uu[e] * d.ldxu + (size_t)sb * R;
-#pragma unroll
-          for (int v = 0; v < NV; ++v)
-            if (live[v / R]) dst[v] = x[v][e];
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < PREP_SPW; ++q) {
-      if (!live[q]) continue;
-#pragma unroll
-      for (int c = 0; c < R; ++c)
-        store_op4<OP>(Ahi, Alo, (size_t)((sb + q) * R + c) * d.ldop + k0, x[q * R + c]);
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-#pragma unroll
-        for (int b = 0; b < R; ++b)
-#pragma unroll
-          for (int a = 0; a <= b; ++a) acc[q * RR + upk(a, b)] += x[q * R + a][e] * x[q * R + b][e];
-    }
-  }
+  if (missing > 0.0) run(true);   // block-uniform; each thread rewrites its own rows
 #pragma unroll
   for (int e = 0; e < PREP_SPW * RR; ++e) {
     const double v = block_sum256(acc[e], shd);
