@@ -58,9 +58,8 @@ struct oagpu_ctx {
   unsigned long long* hash = nullptr;
   int* scratch_i = nullptr;   // rep_of / mismatch
   int G = 0, max_pat_cnt = 0;
-  // Few missing rows in total: the operand pass copies the genotype values
-  // at U (the union of the patterns' missing rows) into Xu and k_corr reads
-  // them there; otherwise k_mask gathers them from the genotypes.
+  // Missing phenotypes: the operand pass copies the genotype values at U (the
+  // union of the patterns' missing rows) into Xu and k_corr reads them there.
   bool compact_corr = false;
   int nU = 0;
   int *uidx = nullptr, *pu_off = nullptr, *pu_list = nullptr;
@@ -594,46 +593,45 @@ extern "C" int oagpu_load_traits(oagpu_ctx* c, const void* Y, int dtype, int whe
   return OAGPU_OK;
 }
 
-// U = union of the patterns' missing rows.  When it is small (at most a
-// quarter of the rows) the corrections use the compact path: uidx, the
-// covariate rows of U and each pattern's rows as positions in U.
+// U = union of the patterns' missing rows, found on the device; uidx maps a
+// row to its position in U, the covariate rows of U are copied, and each
+// pattern's rows are listed as positions in U.  The corrections then read
+// the genotype values at U from the compact u-major buffer the operand pass
+// writes.
 static int build_union(oagpu_ctx* c, const std::vector<int>& pat_rep, const std::vector<int>& pat_cnt) {
   const int n = c->n, l = c->l, G = (int)pat_rep.size();
   c->compact_corr = false;
   c->nU = 0;
-  long long tot = 0;
-  for (int g = 0; g < G; ++g) tot += pat_cnt[g];
-  if (c->max_pat_cnt == 0 || tot > 8ll * n) return OAGPU_OK;
-  std::vector<int> rows_h((size_t)tot);
+  if (c->max_pat_cnt == 0) return OAGPU_OK;
   std::vector<int> off(G + 1, 0);
   for (int g = 0; g < G; ++g) off[g + 1] = off[g] + pat_cnt[g];
-  for (int g = 0; g < G; ++g)
-    if (pat_cnt[g] > 0)
-      HIPCK(c, hipMemcpyAsync(rows_h.data() + off[g], c->rows + (size_t)pat_rep[g] * n, sizeof(int) * pat_cnt[g],
-                              hipMemcpyDeviceToHost, c->st));
+  const size_t tot = (size_t)off[G];
+  HIPCK(c, grow((void**)&c->uidx, &c->uidx_cap, sizeof(int) * (size_t)c->n_pad));
+  HIPCK(c, grow((void**)&c->pu_off, &c->pu_off_cap, sizeof(int) * off.size()));
+  HIPCK(c, grow((void**)&c->pu_list, &c->pu_list_cap, sizeof(int) * std::max<size_t>(tot, 1)));
+  // mark the union in the uidx buffer's bytes, then number it on the host
+  unsigned char* mark = reinterpret_cast<unsigned char*>(c->uidx);
+  HIPCK(c, hipMemsetAsync(mark, 0, (size_t)n, c->st));
+  HIPCK(c, hipMemcpyAsync(c->pu_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice, c->st));
+  launch_union_mark(n, G, c->rows, c->pat_rep, c->pat_cnt, mark, c->st);
+  HIPCK(c, hipGetLastError());
+  std::vector<unsigned char> mk(n);
+  HIPCK(c, hipMemcpyAsync(mk.data(), mark, (size_t)n, hipMemcpyDeviceToHost, c->st));
   HIPCK(c, hipStreamSynchronize(c->st));
   std::vector<int> uidx(c->n_pad, -1);
-  for (int k : rows_h) {
-    if (k < 0 || k >= n) return fail(c, OAGPU_ERR_STATE, "missing-row list out of range");
-    uidx[k] = 0;
-  }
   int nU = 0;
   for (int k = 0; k < n; ++k)
-    if (uidx[k] == 0) uidx[k] = nU++;
-  if (nU > n / 4) return OAGPU_OK;
+    if (mk[k]) uidx[k] = nU++;
   std::vector<double> XLu((size_t)nU * l);
   for (int k = 0; k < n; ++k)
     if (uidx[k] >= 0)
       for (int j = 0; j < l; ++j) XLu[(size_t)uidx[k] * l + j] = c->XL_rm_h[(size_t)k * l + j];
-  for (auto& k : rows_h) k = uidx[k];
-  HIPCK(c, grow((void**)&c->uidx, &c->uidx_cap, sizeof(int) * uidx.size()));
-  HIPCK(c, grow((void**)&c->pu_off, &c->pu_off_cap, sizeof(int) * off.size()));
-  HIPCK(c, grow((void**)&c->pu_list, &c->pu_list_cap, sizeof(int) * std::max<size_t>(rows_h.size(), 1)));
   HIPCK(c, grow((void**)&c->XLu, &c->XLu_cap, sizeof(double) * std::max<size_t>(XLu.size(), 1)));
-  HIPCK(c, hipMemcpy(c->uidx, uidx.data(), sizeof(int) * uidx.size(), hipMemcpyHostToDevice));
-  HIPCK(c, hipMemcpy(c->pu_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
-  HIPCK(c, hipMemcpy(c->pu_list, rows_h.data(), sizeof(int) * rows_h.size(), hipMemcpyHostToDevice));
-  HIPCK(c, hipMemcpy(c->XLu, XLu.data(), sizeof(double) * XLu.size(), hipMemcpyHostToDevice));
+  HIPCK(c, hipMemcpyAsync(c->uidx, uidx.data(), sizeof(int) * uidx.size(), hipMemcpyHostToDevice, c->st));
+  HIPCK(c, hipMemcpyAsync(c->XLu, XLu.data(), sizeof(double) * XLu.size(), hipMemcpyHostToDevice, c->st));
+  launch_union_list(n, G, c->rows, c->pat_rep, c->pat_cnt, c->pu_off, c->uidx, c->pu_list, c->st);
+  HIPCK(c, hipGetLastError());
+  HIPCK(c, hipStreamSynchronize(c->st));   // the host vectors above are released on return
   c->nU = nU;
   c->compact_corr = true;
   return OAGPU_OK;
@@ -755,7 +753,8 @@ static int scan_enqueue(oagpu_ctx* c, bool want_dense) {
   c->use_pst = (long long)c->G * 4 <= t;
   // Chunk size: bounded by max_snps and by ~12 GiB of chunk buffers.
   const size_t per_snp = (size_t)r * c->n_pad * c->op_size * (c->prm.precision == OAGPU_BF16X3 ? 2 : 1) +
-                         (size_t)r * out_cols(c) * c->c_size + sizeof(double) * (size_t)c->Q * std::max(c->G, 1) +
+                         (size_t)r * out_cols(c) * c->c_size +
+                         sizeof(double) * (size_t)c->Q * std::max(c->G, 1) +
                          sizeof(double) * (size_t)(r + r * (r + 1) / 2 + l * r + p) +
                          (c->compact_corr ? sizeof(double) * (size_t)c->nU * r : 0) +
                          (c->use_pst ? sizeof(double) * (size_t)c->G * patsnp_width(r, c->lb) : 0);
@@ -807,7 +806,6 @@ static int scan_enqueue(oagpu_ctx* c, bool want_dense) {
     launch_snp_gram(d, c->prm.precision, c->lb, c->st);
     HIPCK(c, hipEventRecord(E[3], c->st));
     if (c->compact_corr) launch_corr(d, c->lb, c->st);
-    else if (c->max_pat_cnt > 0) launch_mask(d, c->lb, c->st);
     if (c->use_pst) launch_patsnp(d, c->lb, c->st);
     HIPCK(c, hipEventRecord(E[4], c->st));
     launch_reset_counter(c->counters, CNT_FB_CHUNK, c->st);

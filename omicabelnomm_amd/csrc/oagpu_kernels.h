@@ -10,6 +10,12 @@ void launch_ystats(const DevScan& d, int prec, void* Bhi, void* Blo, int* rows_o
                    hipStream_t st);
 void launch_verify_patterns(int t, int n, const int* rows, const int* rep_of, const int* ymiss,
                             int* mismatch, hipStream_t st);
+// union U of the patterns' missing rows (mark), then each pattern's rows as
+// positions in U (uidx: row -> position)
+void launch_union_mark(int n, int G, const int* rows, const int* pat_rep, const int* pat_cnt,
+                       unsigned char* mark, hipStream_t st);
+void launch_union_list(int n, int G, const int* rows, const int* pat_rep, const int* pat_cnt, const int* off,
+                       const int* uidx, int* pu_list, hipStream_t st);
 void launch_pattern(const DevScan& d, const double* Stl_pk, int G, const int* pat_rep, double* UA,
                     double* UA_invd, int* pat_pd, hipStream_t st);
 // zt = U_A^-T XL'y0, ay = U_A^-1 zt (covariate-only coefficients), rss0
@@ -17,11 +23,11 @@ void launch_trait_solve(const DevScan& d, int lb, double* zt, double* ay, double
 
 // SNP side (oagpu_kernels_snp.hip)
 // corr: also write the imputed values at the rows of U (d.Xu), read by
-// launch_corr (used when the patterns' missing rows are few).
+// launch_corr.
 void launch_prep(const DevScan& d, int prec, void* Ahi, void* Alo, bool corr, hipStream_t st);
 void launch_snp_gram(const DevScan& d, int prec, int lb, hipStream_t st);
-void launch_mask(const DevScan& d, int lb, hipStream_t st);
 void launch_corr(const DevScan& d, int lb, hipStream_t st);
+// one thread per pair, reading the corrections from d.Ccorr
 void launch_epilogue(const DevScan& d, int prec, int lb, hipStream_t st);
 // Few patterns: per-(SNP, pattern) solve pieces (d.pst, patsnp_width
 // doubles each), then the per-pair epilogue that reads them.

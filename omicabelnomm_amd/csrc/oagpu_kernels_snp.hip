@@ -312,76 +312,21 @@ __global__ void k_snp_gram(DevScan d) {
 }
 
 // ---------------------------------------------------------------------------
-// Missing-row corrections per (pattern, SNP): C_lr = sum_M xl x_s' (l x r)
-// and C_rr = sum_M x_s x_s' (upper), over the pattern's missing rows.  One
-// wave per pattern, one lane per SNP; the row index and covariate row are
-// wave-uniform.
+// generate_correction_missings_Str (src/Algorithm.cpp:1260-1315) for every
+// pattern: C_lr = sum over the pattern's missing rows of xl x', C_rr of x x'.
+// The genotype values come from the compact u-major rows of U written by the
+// operand pass.  One wave per pattern, one lane per SNP: the pattern's row
+// positions and covariate rows are wave-uniform, and the 64 SNPs' values at
+// one row are contiguous.  Blocks are numbered so that the pattern groups of
+// one 64-SNP tile run back to back on one XCD and share its L2.
 template <int R, int LB>
-__global__ __launch_bounds__(256) void k_mask(DevScan d) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int g = blockIdx.y * 4 + w;
-  if (g >= d.G) return;
-  const int s = blockIdx.x * 64 + lane;
-  const int se = s < d.mc ? s : d.mc - 1;
-  const int l = d.l;
-  constexpr int RR = R * (R + 1) / 2;
-  int src[R], env[R];
-  col_desc<R>(d, d.s0 + se, src, env);
-  double mean[R];
-#pragma unroll
-  for (int c = 0; c < R; ++c) mean[c] = d.col_mean[(size_t)se * R + c];
-  double lr[LB][R], rr[RR];
-#pragma unroll
-  for (int j = 0; j < LB; ++j)
-#pragma unroll
-    for (int c = 0; c < R; ++c) lr[j][c] = 0.0;
-#pragma unroll
-  for (int e = 0; e < RR; ++e) rr[e] = 0.0;
-  const int cnt = d.pat_cnt[g];
-  const int* rows = d.rows + (size_t)d.pat_rep[g] * d.n;
-  for (int q = 0; q < cnt; ++q) {
-    const int k = rows[q];
-    double x[R];
-#pragma unroll
-    for (int c = 0; c < R; ++c) {
-      double v = geno_value(d, src[c], env[c], k);
-      x[c] = (v == v) ? v : mean[c];
-    }
-    const double* xl = d.XL_rm + (size_t)k * l;
-#pragma unroll
-    for (int j = 0; j < LB; ++j)
-      if (j < l) {
-        const double a = xl[j];
-#pragma unroll
-        for (int c = 0; c < R; ++c) lr[j][c] += a * x[c];
-      }
-#pragma unroll
-    for (int b = 0; b < R; ++b)
-#pragma unroll
-      for (int a = 0; a <= b; ++a) rr[upk(a, b)] += x[a] * x[b];
-  }
-  if (s < d.mc) {
-    const size_t base = (size_t)s * d.Q;
-#pragma unroll
-    for (int j = 0; j < LB; ++j)
-      if (j < l)
-#pragma unroll
-        for (int c = 0; c < R; ++c) d.Ccorr[(base + j * R + c) * d.G + g] = lr[j][c];
-#pragma unroll
-    for (int e = 0; e < RR; ++e) d.Ccorr[(base + l * R + e) * d.G + g] = rr[e];
-  }
-}
-
-// Corrections from the compact rows of U written by the operand pass: one
-// wave per pattern, one lane per SNP; the pattern's rows and covariate rows
-// are wave-uniform, the genotype values of 64 consecutive SNPs at one row are
-// contiguous.  Same sums as k_mask (src/Algorithm.cpp:1260-1315).
-template <int R, int LB>
-__global__ __launch_bounds__(256) void k_corr(DevScan d) {
+__global__ __launch_bounds__(256) void k_corr(DevScan d, int gq_count) {
   const int lane = threadIdx.x & 63;
-  const int g = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + (threadIdx.x >> 6));
-  if (g >= d.G) return;
-  const int s = blockIdx.x * 64 + lane;
+  const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
+  const int tile = (local / gq_count) * 8 + xcd;
+  const int g = __builtin_amdgcn_readfirstlane((local % gq_count) * 4 + (threadIdx.x >> 6));
+  if (g >= d.G || tile * 64 >= d.mc) return;
+  const int s = tile * 64 + lane;
   const int se = s < d.mc ? s : d.mc - 1;
   const int l = d.l;
   constexpr int RR = R * (R + 1) / 2;
@@ -434,7 +379,10 @@ __device__ __forceinline__ double pvalue_ref(double t) {
   return erfc(t * 0.70710678118654752440084436210484903928);
 }
 
-template <int P>
+// Coefficient vectors use a split layout: b[0..LB) covariates (index h < l
+// valid), b[LB..LB+R) the SNP terms (coefficient index h = l + c), so every
+// register index is a compile-time constant.
+template <int R, int LB>
 __device__ void stats_emit(const DevScan& d, int y, int s, const double* b, double rss) {
   const int l = d.l, p = d.p;
   const int ymiss = d.ymiss[y];
@@ -449,15 +397,18 @@ __device__ void stats_emit(const DevScan& d, int y, int s, const double* b, doub
     d.d_r2[pair] = R2;
     d.d_rss[pair] = rss;
 #pragma unroll
-    for (int h = 0; h < P; ++h)
-      if (h < p) {
-        d.d_beta[pair * p + h] = b[h];
+    for (int j = 0; j < LB + R; ++j) {
+      const int h = j < LB ? j : l + (j - LB);
+      if (j >= LB || j < l) {
+        d.d_beta[pair * p + h] = b[j];
         d.d_se[pair * p + h] = qnan;
         d.d_t[pair * p + h] = qnan;
         d.d_p[pair * p + h] = qnan;
         d.d_kept[pair * p + h] = 0;
       }
+    }
   }
+  // kept[j] for slot j of the split layout
   unsigned long long mask = 0;
   int nk = 0, nsig = 0;
   bool has_snp = false;
@@ -467,31 +418,30 @@ __device__ void stats_emit(const DevScan& d, int y, int s, const double* b, doub
     const double s2 = rss / (double)n_c * vf;
     const double tt = d.t_star * d.t_star;   // exact: t_star is a float
 #pragma unroll
-    for (int h = P - 1; h >= 0; --h) {
-      if (h < p && h >= d.start_h) {
-        const double vxh = vx[h];
-        if (vxh < 0.0) atomicOr(&d.counters[CNT_ERR], 1ull);
-        // |b/se| > t_star  <=>  b^2 > t_star^2 se^2 (se^2 >= 0): the
-        // square roots and the division are only needed for what is kept
-        const double se2 = s2 * vxh;
-        const bool pass = dense || d.store_all || (se2 >= 0.0 && b[h] * b[h] > tt * se2);
-        if (!pass) continue;
-        const double se = sig * sqrt(vxh * vf);
-        const double tv = fabs(b[h] / se);
+    for (int j = 0; j < LB + R; ++j) {
+      const int h = j < LB ? j : l + (j - LB);
+      if (j < LB && (j >= l || j < d.start_h)) continue;
+      const double vxh = vx[h];
+      if (vxh < 0.0) atomicOr(&d.counters[CNT_ERR], 1ull);
+      // |b/se| > t_star  <=>  b^2 > t_star^2 se^2 (se^2 >= 0): the square
+      // roots and the division are only needed for what is kept
+      const double se2 = s2 * vxh;
+      if (!(dense || d.store_all || (se2 >= 0.0 && b[j] * b[j] > tt * se2))) continue;
+      const double se = sig * sqrt(vxh * vf);
+      const double tv = fabs(b[j] / se);
+      if (dense) {
+        d.d_se[pair * p + h] = se;
+        d.d_t[pair * p + h] = tv;
+      }
+      if (tv > d.t_star || d.store_all) {
+        mask |= 1ull << j;
+        ++nk;
+        if (j >= LB) has_snp = true;
+        const double pv = pvalue_ref(tv);
+        if (d.min_p_disp > pv) ++nsig;
         if (dense) {
-          d.d_se[pair * p + h] = se;
-          d.d_t[pair * p + h] = tv;
-        }
-        if (tv > d.t_star || d.store_all) {
-          mask |= 1ull << h;
-          ++nk;
-          if (h >= l) has_snp = true;
-          const double pv = pvalue_ref(tv);
-          if (d.min_p_disp > pv) ++nsig;
-          if (dense) {
-            d.d_p[pair * p + h] = pv;
-            d.d_kept[pair * p + h] = 1;
-          }
+          d.d_p[pair * p + h] = pv;
+          d.d_kept[pair * p + h] = 1;
         }
       }
     }
@@ -503,23 +453,25 @@ __device__ void stats_emit(const DevScan& d, int y, int s, const double* b, doub
   const unsigned long long base = atomicAdd(&d.counters[CNT_HITS], (unsigned long long)nk);
   if (base + nk > (unsigned long long)d.hit_cap) return;
   unsigned long long o = base;
+  // SNP terms then covariates, each descending: h descending overall
 #pragma unroll
-  for (int h = P - 1; h >= 0; --h) {
-    if (h < p && ((mask >> h) & 1ull)) {
-      const double se = sig * sqrt(vx[h] * vf);
-      const double tv = fabs(b[h] / se);
-      oagpu_hit hr;
-      hr.trait = d.trait_offset + y;
-      hr.snp = d.snp_offset + d.s0 + s;
-      hr.h = h;
-      hr.n_used = d.n - ymiss;
-      hr.b = b[h];
-      hr.se = se;
-      hr.t = tv;
-      hr.p = pvalue_ref(tv);
-      hr.r2 = R2;
-      d.hits[o++] = hr;
-    }
+  for (int jj = 0; jj < LB + R; ++jj) {
+    const int j = jj < R ? LB + R - 1 - jj : LB - 1 - (jj - R);
+    if (!((mask >> j) & 1ull)) continue;
+    const int h = j < LB ? j : l + (j - LB);
+    const double se = sig * sqrt(vx[h] * vf);
+    const double tv = fabs(b[j] / se);
+    oagpu_hit hr;
+    hr.trait = d.trait_offset + y;
+    hr.snp = d.snp_offset + d.s0 + s;
+    hr.h = h;
+    hr.n_used = d.n - ymiss;
+    hr.b = b[j];
+    hr.se = se;
+    hr.t = tv;
+    hr.p = pvalue_ref(tv);
+    hr.r2 = R2;
+    d.hits[o++] = hr;
   }
 }
 
@@ -544,7 +496,7 @@ __device__ double explicit_rss_thread(const DevScan& d, int y, int s, const doub
     for (int c = 0; c < R; ++c) {
       double v = geno_value(d, src[c], env[c], k);
       if (v != v) v = mean[c];
-      sy += v * b[d.l + c];
+      sy += v * b[LB + c];
     }
     sy -= yv;
     acc += sy * sy;
@@ -552,39 +504,19 @@ __device__ double explicit_rss_thread(const DevScan& d, int y, int s, const doub
   return acc;
 }
 
-// Shared end of a pair: the compact coefficient vector, the hand-over to the
-// explicit residual pass, and the statistics.  b[0..LB) covariates,
-// b[LB..LB+R) SNP terms.
+// Shared end of a pair: the hand-over to the explicit residual pass and the
+// statistics.  b in the split layout (b[0..LB) covariates, b[LB..LB+R) SNP
+// terms).
 template <int R, int LB>
-__device__ __forceinline__ void finish_pair(const DevScan& d, int y, int s, bool ok, const double* b,
+__device__ __forceinline__ void finish_pair(const DevScan& d, int y, int s, bool ok, double* b,
                                             const double* rb, double rss) {
-  constexpr int P = LB + R;
-  const int l = d.l;
-  // Compact coefficient vector: b[0..l) covariates, b[l..l+R) SNP terms.
-  double bc[P];
-#pragma unroll
-  for (int h = 0; h < P; ++h) bc[h] = 0.0;
-#pragma unroll
-  for (int h = 0; h < LB; ++h)
-    if (h < l) bc[h] = b[h];
-#pragma unroll
-  for (int c = 0; c < R; ++c) {
-#pragma unroll
-    for (int h = 0; h < P; ++h)
-      if (h == l + c) bc[h] = b[LB + c];
-  }
   const bool fallback = !ok || d.yzero[y] > 0 || !(rss >= RSS_GUARD * d.yy[y]);
   if (fallback) {
     if (!ok) {   // sposv left the right-hand side in place
 #pragma unroll
-      for (int h = 0; h < P; ++h) {
-        if (h < l) bc[h] = d.rt[(size_t)h * d.ldt + y];
-      }
+      for (int h = 0; h < LB; ++h) b[h] = (h < d.l) ? d.rt[(size_t)h * d.ldt + y] : 0.0;
 #pragma unroll
-      for (int c = 0; c < R; ++c)
-#pragma unroll
-        for (int h = 0; h < P; ++h)
-          if (h == l + c) bc[h] = rb[c];
+      for (int c = 0; c < R; ++c) b[LB + c] = rb[c];
     }
     const unsigned long long slot = atomicAdd(&d.counters[CNT_FB_CHUNK], 1ull);
     atomicAdd(&d.counters[CNT_FALLBACK], 1ull);
@@ -594,12 +526,12 @@ __device__ __forceinline__ void finish_pair(const DevScan& d, int y, int s, bool
       f->s = s;
       f->flags = ok ? 0 : 1;
 #pragma unroll
-      for (int h = 0; h < P; ++h) f->b[h] = bc[h];
+      for (int h = 0; h < LB + R; ++h) f->b[h] = b[h];
       return;
     }
-    rss = explicit_rss_thread<R, LB>(d, y, s, bc);
+    rss = explicit_rss_thread<R, LB>(d, y, s, b);
   }
-  stats_emit<P>(d, y, s, bc, rss);
+  stats_emit<R, LB>(d, y, s, b, rss);
 }
 
 // ---------------------------------------------------------------------------
@@ -1077,15 +1009,13 @@ __global__ __launch_bounds__(256) void k_fallback(DevScan d) {
       for (int c = 0; c < R; ++c) {
         double v = geno_value(d, src[c], env[c], k);
         if (v != v) v = mean[c];
-#pragma unroll
-        for (int h = 0; h < P; ++h)
-          if (h == d.l + c) sy += v * b[h];
+        sy += v * b[LB + c];
       }
       sy -= yv;
       acc += sy * sy;
     }
     acc = wave_sum(acc);
-    if (lane == 0) stats_emit<P>(d, y, s, b, acc);
+    if (lane == 0) stats_emit<R, LB>(d, y, s, b, acc);
   }
 }
 
@@ -1133,16 +1063,10 @@ void launch_snp_gram(const DevScan& d, int prec, int lb, hipStream_t st) {
 #undef OA_SG
 }
 
-void launch_mask(const DevScan& d, int lb, hipStream_t st) {
-  dim3 grid((d.mc + 63) / 64, (d.G + 3) / 4), block(256);
-#define OA_MK(R_, LB_) hipLaunchKernelGGL((k_mask<R_, LB_>), grid, block, 0, st, d);
-  OA_DISPATCH_RLB(0, 0, OA_MK)
-#undef OA_MK
-}
-
 void launch_corr(const DevScan& d, int lb, hipStream_t st) {
-  dim3 grid((d.mc + 63) / 64, (d.G + 3) / 4), block(256);
-#define OA_CR(R_, LB_) hipLaunchKernelGGL((k_corr<R_, LB_>), grid, block, 0, st, d);
+  const int tiles = (d.mc + 63) / 64, gq = (d.G + 3) / 4;
+  dim3 grid((unsigned)(((tiles + 7) / 8) * 8 * gq)), block(256);
+#define OA_CR(R_, LB_) hipLaunchKernelGGL((k_corr<R_, LB_>), grid, block, 0, st, d, gq);
   OA_DISPATCH_RLB(0, 0, OA_CR)
 #undef OA_CR
 }

@@ -113,6 +113,25 @@ __global__ void k_verify_patterns(int t, int n, const int* rows, const int* rep_
   mismatch[y] = bad;
 }
 
+// Union of the patterns' missing rows: mark[k] = 1 for every row some
+// pattern misses (every writer stores the same value).
+__global__ void k_union_mark(int n, const int* rows, const int* pat_rep, const int* pat_cnt,
+                             unsigned char* mark) {
+  const int g = blockIdx.x;
+  const int cnt = pat_cnt[g];
+  const int* r = rows + (size_t)pat_rep[g] * n;
+  for (int i = threadIdx.x; i < cnt; i += blockDim.x) mark[r[i]] = 1;
+}
+
+// Pattern g's missing rows as positions in the union, at pu_list + off[g].
+__global__ void k_union_list(int n, const int* rows, const int* pat_rep, const int* pat_cnt,
+                             const int* off, const int* uidx, int* pu_list) {
+  const int g = blockIdx.x;
+  const int cnt = pat_cnt[g];
+  const int* r = rows + (size_t)pat_rep[g] * n;
+  for (int i = threadIdx.x; i < cnt; i += blockDim.x) pu_list[off[g] + i] = uidx[r[i]];
+}
+
 // One block per pattern: C_ll = sum over the pattern's missing rows of
 // xl xl', A = Stl - C_ll, then the upper Cholesky factor of A with LAPACK's
 // failure rule (a pivot <= 0 or NaN).
@@ -213,6 +232,16 @@ void launch_verify_patterns(int t, int n, const int* rows, const int* rep_of, co
                             int* mismatch, hipStream_t st) {
   hipLaunchKernelGGL(k_verify_patterns, dim3((t + 255) / 256), dim3(256), 0, st, t, n, rows, rep_of,
                      ymiss, mismatch);
+}
+
+void launch_union_mark(int n, int G, const int* rows, const int* pat_rep, const int* pat_cnt,
+                       unsigned char* mark, hipStream_t st) {
+  hipLaunchKernelGGL(k_union_mark, dim3(G), dim3(256), 0, st, n, rows, pat_rep, pat_cnt, mark);
+}
+
+void launch_union_list(int n, int G, const int* rows, const int* pat_rep, const int* pat_cnt, const int* off,
+                       const int* uidx, int* pu_list, hipStream_t st) {
+  hipLaunchKernelGGL(k_union_list, dim3(G), dim3(256), 0, st, n, rows, pat_rep, pat_cnt, off, uidx, pu_list);
 }
 
 void launch_pattern(const DevScan& d, const double* Stl_pk, int G, const int* pat_rep, double* UA,
