@@ -117,6 +117,15 @@ int oagpu_create(const oagpu_params *params, oagpu_ctx **ctx);
 void oagpu_destroy(oagpu_ctx *ctx);
 const char *oagpu_last_error(const oagpu_ctx *ctx);
 
+/* Individual subset: keep[k] != 0 keeps individual k (n entries; NULL
+ * keeps all).  Rows outside the subset take no part in any sum, count or
+ * statistic, as if the reference had excluded them (src/AIOwrapper.cpp:
+ * 1947-2233), so one loaded panel serves analyses of different subsets.
+ * Call before oagpu_set_covariates; it clears the loaded covariates and
+ * traits.  No counterpart call in the reference, which removes excluded
+ * rows while reading the files. */
+int oagpu_set_subset(oagpu_ctx *ctx, const uint8_t *keep);
+
 /* Covariates XL (n x l, FP64 host, no NaN).  Runs the reference's rank
  * check (src/Algorithm.cpp:497-527) and returns OAGPU_ERR_COVARIATES when
  * a singular value is below 1e-6. */

@@ -16,143 +16,9 @@
 #include <tuple>
 #include <vector>
 
-#include "oagpu_kernels.h"
+#include "oagpu_ctx.h"
 
 using namespace oa;
-
-struct oagpu_ctx {
-  oagpu_params prm{};
-  hipStream_t st = nullptr;
-  std::string err;
-  int n = 0, n_pad = 0, l = 0, r = 0, p = 0, Q = 0, lb = 0;
-  size_t op_size = 8, c_size = 8;
-  // Row stride (elements) of the contraction operands A and B.  Split-BF16
-  // rows interleave the two halves: A rows [hi | lo], B rows [lo | hi].
-  long long ldop = 0;
-
-  // covariates
-  bool have_cov = false;
-  std::vector<double> Stl_inv_diag;
-  double *XL_cm = nullptr, *XL_rm = nullptr, *U0 = nullptr, *Stl_inv_d = nullptr, *Stl_pk = nullptr;
-  std::vector<double> XL_rm_h;   // [n][l]
-  void *XLop_hi = nullptr, *XLop_lo = nullptr;   // covariate rows of B, in the operand format
-
-  // interactions
-  double* E = nullptr;
-  long long lde = 0;
-  int n_env = 0;
-
-  // traits
-  bool have_traits = false;
-  int t = 0, ldt = 0, trait_offset = 0;
-  const void* Y = nullptr;
-  int ydt = 0;
-  long long ldy = 0;
-  void* Y_own = nullptr;
-  size_t Y_own_bytes = 0;
-  void* Bbuf = nullptr;
-  void *Bhi = nullptr, *Blo = nullptr;   // views into Bbuf
-  int* rows = nullptr;
-  int *ymiss = nullptr, *yzero = nullptr, *ypat = nullptr;
-  double *yy = nullptr, *sst = nullptr, *rss0 = nullptr, *rt = nullptr, *zt = nullptr, *ay = nullptr;
-  unsigned long long* hash = nullptr;
-  int* scratch_i = nullptr;   // rep_of / mismatch
-  int G = 0, max_pat_cnt = 0;
-  // Missing phenotypes: the operand pass copies the genotype values at U (the
-  // union of the patterns' missing rows) into Xu and k_corr reads them there.
-  bool compact_corr = false;
-  int nU = 0;
-  int *uidx = nullptr, *pu_off = nullptr, *pu_list = nullptr;
-  double* XLu = nullptr;
-  size_t uidx_cap = 0, pu_off_cap = 0, pu_list_cap = 0, XLu_cap = 0;
-  int *pat_cnt = nullptr, *pat_rep = nullptr, *pat_pd = nullptr;
-  double *UA = nullptr, *UA_invd = nullptr;
-  size_t pat_cap = 0;
-
-  // genotypes: host blocks are uploaded on a second stream into one of two
-  // device slots, so the next block's copy overlaps the current scan
-  bool have_geno = false;
-  const void* Gm = nullptr;
-  int gdt = 0;
-  long long ldg = 0;
-  int n_src = 0, m = 0, snp_offset = 0;
-  hipStream_t cst = nullptr;
-  void* G_slot[2] = {nullptr, nullptr};
-  size_t G_slot_cap[2] = {0, 0};
-  int* d_src[2] = {nullptr, nullptr};
-  int* d_env[2] = {nullptr, nullptr};
-  size_t d_desc_cap[2] = {0, 0};
-  std::vector<int> h_desc[2];
-  hipEvent_t up_ev[2] = {nullptr, nullptr}, free_ev[2] = {nullptr, nullptr};
-  bool free_valid[2] = {false, false};
-  int gslot = 1;
-  bool upload_pending = false;
-  int *col_src = nullptr, *col_env = nullptr;
-
-  // a scan in flight (oagpu_scan_async .. oagpu_scan_wait)
-  bool pending = false;
-  bool p_dense = false;
-  int p_nchunks = 0, p_t = 0, p_m = 0;
-  std::vector<hipEvent_t> pev;
-  hipEvent_t done_ev = nullptr;
-  unsigned long long* cnt_h = nullptr;   // pinned copy of the counters
-  double *pd_beta = nullptr, *pd_se = nullptr, *pd_t = nullptr, *pd_p = nullptr, *pd_r2 = nullptr,
-         *pd_rss = nullptr, *pd_varx = nullptr;
-  uint8_t *pd_kept = nullptr, *pd_em = nullptr;
-
-  // chunk buffers
-  int mc_cap = 0;
-  void* Abuf = nullptr;
-  void *Ahi = nullptr, *Alo = nullptr, *Cbuf = nullptr;   // Ahi, Alo: views into Abuf
-  double *col_mean = nullptr, *Sbr = nullptr, *Str = nullptr, *varx = nullptr, *Ccorr = nullptr;
-  double* Xu = nullptr;
-  long long ldxu = 0;
-  // few patterns per trait block: per-(SNP, pattern) solve pieces
-  bool use_pst = false;
-  double* pst = nullptr;
-  size_t A_cap = 0, C_cap = 0, Ccorr_cap = 0, Xu_cap = 0, pst_cap = 0;
-
-  // outputs
-  oagpu_hit* hits = nullptr;
-  long long hit_cap = 0;
-  unsigned long long* counters = nullptr;
-  FallbackEntry* fb = nullptr;
-  int fb_cap = 0;
-  double timings[8] = {0};
-  std::string gemm_variant = "none";
-};
-
-#define HIPCK(ctx, expr)                                                              \
-  do {                                                                                \
-    hipError_t e_ = (expr);                                                           \
-    if (e_ != hipSuccess) {                                                           \
-      (ctx)->err = std::string("HIP error: ") + hipGetErrorString(e_) + " at " #expr; \
-      return OAGPU_ERR_HIP;                                                           \
-    }                                                                                 \
-  } while (0)
-
-static int fail(oagpu_ctx* c, int code, const std::string& msg) {
-  c->err = msg;
-  return code;
-}
-
-// (Re)allocate a device buffer so it holds at least `bytes`.
-static hipError_t grow(void** p, size_t* cap, size_t bytes) {
-  if (*p && *cap >= bytes) return hipSuccess;
-  if (*p) {
-    hipError_t e = hipFree(*p);
-    if (e != hipSuccess) return e;
-    *p = nullptr;
-  }
-  *cap = 0;
-  hipError_t e = hipMalloc(p, bytes > 0 ? bytes : 64);
-  if (e == hipSuccess) *cap = bytes;
-  return e;
-}
-
-static void dfree(void* p) {
-  if (p) (void)hipFree(p);
-}
 
 static double erfinv_host(double x) {
   if (x <= -1.0) return -INFINITY;
@@ -243,11 +109,12 @@ static double min_singular(const std::vector<double>& S, int l) {
 // Row stride of the contraction output: traits then covariates, padded to
 // 64 columns (the padding columns are computed from zero or stale B rows and
 // never read), so library GEMMs see aligned shapes.
-static int out_cols(const oagpu_ctx* c) { return (c->t + c->l + 63) / 64 * 64; }
+int oa::out_cols(const oagpu_ctx* c) { return (c->t + c->l + 63) / 64 * 64; }
 
-static DevScan make_dev(const oagpu_ctx* c) {
+DevScan oa::make_dev(const oagpu_ctx* c) {
   DevScan d{};
-  d.n = c->n; d.n_pad = c->n_pad; d.ldop = c->ldop; d.l = c->l; d.r = c->r; d.p = c->p; d.Q = c->Q;
+  d.n = c->n; d.n_pad = c->n_pad; d.ldop = c->ldop;
+  d.keep = c->keep; d.n_stat = c->keep ? c->n_stat : c->n; d.l = c->l; d.r = c->r; d.p = c->p; d.Q = c->Q;
   d.t = c->t; d.ldt = c->ldt; d.ldc = out_cols(c); d.G = c->G;
   d.start_h = c->prm.disp_cov ? 1 : c->l;
   d.disp_cov = c->prm.disp_cov; d.store_all = c->prm.store_all; d.need_bl = c->prm.disp_cov;
@@ -360,7 +227,7 @@ extern "C" void oagpu_destroy(oagpu_ctx* c) {
   for (void* b : dense_bufs) dfree(b);
   void* bufs[] = {c->XL_cm, c->XL_rm, c->U0, c->Stl_inv_d, c->Stl_pk, c->XLop_hi, c->XLop_lo, c->E,
                   c->Y_own, c->Bbuf, c->rows, c->ymiss, c->yzero, c->ypat, c->yy, c->sst,
-                  c->rss0, c->rt, c->zt, c->ay, c->hash, c->scratch_i, c->pat_cnt, c->pat_rep, c->pat_pd,
+                  c->rss0, c->rt, c->zt, c->ay, c->hash, c->scratch_i, c->pat_cnt, c->pat_rep, c->pat_pd, c->keep,
                   c->UA, c->UA_invd, c->Abuf, c->Cbuf,
                   c->col_mean, c->Sbr, c->Str, c->varx, c->Ccorr, c->hits, c->counters, c->fb,
                   c->uidx, c->pu_off, c->pu_list, c->XLu, c->Xu, c->pst};
@@ -372,11 +239,48 @@ extern "C" void oagpu_destroy(oagpu_ctx* c) {
 
 extern "C" const char* oagpu_last_error(const oagpu_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
+extern "C" int oagpu_set_subset(oagpu_ctx* c, const uint8_t* keep) {
+  if (!c) return OAGPU_ERR_ARG;
+  if (c->pending) return fail(c, OAGPU_ERR_STATE, "a scan is in flight (oagpu_scan_wait first)");
+  HIPCK(c, hipSetDevice(c->prm.device));
+  c->have_cov = c->have_traits = false;
+  if (!keep) {
+    c->keep_h.clear();
+    dfree(c->keep);
+    c->keep = nullptr;
+    c->n_stat = c->n;
+    return OAGPU_OK;
+  }
+  c->keep_h.assign(keep, keep + c->n);
+  int nk = 0;
+  for (auto& v : c->keep_h) {
+    v = v ? 1 : 0;
+    nk += v;
+  }
+  if (nk <= c->p + 1) return fail(c, OAGPU_ERR_ARG, "the subset holds too few individuals");
+  if (!c->keep) HIPCK(c, hipMalloc(&c->keep, (size_t)c->n_pad));
+  HIPCK(c, hipMemset(c->keep, 0, (size_t)c->n_pad));
+  HIPCK(c, hipMemcpy(c->keep, c->keep_h.data(), (size_t)c->n, hipMemcpyHostToDevice));
+  c->n_stat = nk;
+  return OAGPU_OK;
+}
+
 extern "C" int oagpu_set_covariates(oagpu_ctx* c, const double* XL, long long ld) {
   if (!c || !XL || ld < c->n) return c ? fail(c, OAGPU_ERR_ARG, "bad covariates") : OAGPU_ERR_ARG;
   if (c->pending) return fail(c, OAGPU_ERR_STATE, "a scan is in flight (oagpu_scan_wait first)");
   HIPCK(c, hipSetDevice(c->prm.device));
   const int n = c->n, l = c->l, np = c->n_pad;
+  // with a subset, the rows outside it are zero (they then drop out of
+  // every covariate product)
+  std::vector<double> XLm;
+  if (!c->keep_h.empty()) {
+    XLm.assign((size_t)n * l, 0.0);
+    for (int j = 0; j < l; ++j)
+      for (int k = 0; k < n; ++k)
+        if (c->keep_h[k]) XLm[(size_t)j * n + k] = XL[(size_t)j * ld + k];
+    XL = XLm.data();
+    ld = n;
+  }
   std::vector<double> S(l * l);
   for (int i = 0; i < l; ++i)
     for (int j = 0; j < l; ++j) {
@@ -609,15 +513,17 @@ static int build_union(oagpu_ctx* c, const std::vector<int>& pat_rep, const std:
   HIPCK(c, grow((void**)&c->uidx, &c->uidx_cap, sizeof(int) * (size_t)c->n_pad));
   HIPCK(c, grow((void**)&c->pu_off, &c->pu_off_cap, sizeof(int) * off.size()));
   HIPCK(c, grow((void**)&c->pu_list, &c->pu_list_cap, sizeof(int) * std::max<size_t>(tot, 1)));
-  // mark the union in the uidx buffer's bytes, then number it on the host
-  unsigned char* mark = reinterpret_cast<unsigned char*>(c->uidx);
-  HIPCK(c, hipMemsetAsync(mark, 0, (size_t)n, c->st));
   HIPCK(c, hipMemcpyAsync(c->pu_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice, c->st));
-  launch_union_mark(n, G, c->rows, c->pat_rep, c->pat_cnt, mark, c->st);
-  HIPCK(c, hipGetLastError());
-  std::vector<unsigned char> mk(n);
-  HIPCK(c, hipMemcpyAsync(mk.data(), mark, (size_t)n, hipMemcpyDeviceToHost, c->st));
-  HIPCK(c, hipStreamSynchronize(c->st));
+  std::vector<unsigned char> mk(n, 1);
+  if (!c->union_identity) {
+    // mark the union in the uidx buffer's bytes, then number it on the host
+    unsigned char* mark = reinterpret_cast<unsigned char*>(c->uidx);
+    HIPCK(c, hipMemsetAsync(mark, 0, (size_t)n, c->st));
+    launch_union_mark(n, G, c->rows, c->pat_rep, c->pat_cnt, mark, c->st);
+    HIPCK(c, hipGetLastError());
+    HIPCK(c, hipMemcpyAsync(mk.data(), mark, (size_t)n, hipMemcpyDeviceToHost, c->st));
+    HIPCK(c, hipStreamSynchronize(c->st));
+  }
   std::vector<int> uidx(c->n_pad, -1);
   int nU = 0;
   for (int k = 0; k < n; ++k)
@@ -714,15 +620,9 @@ extern "C" void oagpu_host_free(void* ptr) {
   if (ptr) (void)hipHostFree(ptr);
 }
 
-static int ensure_chunk(oagpu_ctx* c, int mc) {
+int oa::ensure_pair_buffers(oagpu_ctx* c, int mc) {
   const int r = c->r, l = c->l, p = c->p, G = c->G, Q = c->Q;
-  const size_t pad = (size_t)gemm_row_pad();
-  const size_t arows = ((size_t)mc * r + pad - 1) / pad * pad;
-  const size_t abytes = arows * c->ldop * c->op_size;
-  const size_t cbytes = (size_t)mc * r * out_cols(c) * c->c_size;
   const size_t qbytes = sizeof(double) * (size_t)mc * Q * std::max(G, 1);
-  c->ldxu = ((long long)mc * r + 7) / 8 * 8;
-  const size_t xubytes = c->compact_corr ? sizeof(double) * (size_t)c->nU * c->ldxu : 0;
   if (mc > c->mc_cap) {
     dfree(c->col_mean); dfree(c->Sbr); dfree(c->Str); dfree(c->varx);
     HIPCK(c, hipMalloc(&c->col_mean, sizeof(double) * (size_t)mc * r));
@@ -731,15 +631,28 @@ static int ensure_chunk(oagpu_ctx* c, int mc) {
     HIPCK(c, hipMalloc(&c->varx, sizeof(double) * (size_t)mc * p));
     c->mc_cap = mc;
   }
+  HIPCK(c, grow((void**)&c->Ccorr, &c->Ccorr_cap, qbytes));
+  if (c->use_pst)
+    HIPCK(c, grow((void**)&c->pst, &c->pst_cap,
+                  sizeof(double) * (size_t)mc * G * patsnp_width(r, c->lb)));
+  return OAGPU_OK;
+}
+
+static int ensure_chunk(oagpu_ctx* c, int mc) {
+  const int r = c->r;
+  const size_t pad = (size_t)gemm_row_pad();
+  const size_t arows = ((size_t)mc * r + pad - 1) / pad * pad;
+  const size_t abytes = arows * c->ldop * c->op_size;
+  const size_t cbytes = (size_t)mc * r * out_cols(c) * c->c_size;
+  c->ldxu = ((long long)mc * r + 7) / 8 * 8;
+  const size_t xubytes = c->compact_corr ? sizeof(double) * (size_t)c->nU * c->ldxu : 0;
+  int rc = ensure_pair_buffers(c, mc);
+  if (rc) return rc;
   HIPCK(c, grow(&c->Abuf, &c->A_cap, abytes));
   c->Ahi = c->Abuf;
   c->Alo = c->prm.precision == OAGPU_BF16X3 ? static_cast<uint16_t*>(c->Abuf) + c->n_pad : nullptr;
   HIPCK(c, grow(&c->Cbuf, &c->C_cap, cbytes));
-  HIPCK(c, grow((void**)&c->Ccorr, &c->Ccorr_cap, qbytes));
   if (xubytes) HIPCK(c, grow((void**)&c->Xu, &c->Xu_cap, xubytes));
-  if (c->use_pst)
-    HIPCK(c, grow((void**)&c->pst, &c->pst_cap,
-                  sizeof(double) * (size_t)mc * G * patsnp_width(r, c->lb)));
   return OAGPU_OK;
 }
 
@@ -883,7 +796,7 @@ static int scan_collect(oagpu_ctx* c, oagpu_dense* dense, oagpu_hit* hits, long 
       HIPCK(c, cp(dense->var_x, c->pd_varx, sizeof(double) * (size_t)m * p));
       HIPCK(c, cp(dense->n_used, c->ymiss, sizeof(int) * t));
       if (dense->n_used)
-        for (int y = 0; y < t; ++y) dense->n_used[y] = c->n - dense->n_used[y];
+        for (int y = 0; y < t; ++y) dense->n_used[y] = (c->keep ? c->n_stat : c->n) - dense->n_used[y];
     }
     void* bufs[] = {c->pd_beta, c->pd_se, c->pd_t, c->pd_p, c->pd_kept, c->pd_r2, c->pd_rss, c->pd_em, c->pd_varx};
     for (void* b : bufs) dfree(b);

@@ -29,6 +29,8 @@ struct DevScan {
   // shapes
   int n, n_pad, l, r, p, Q;   // Q = l*r + r*(r+1)/2 correction terms
   long long ldop;             // row stride of the contraction operands (elements)
+  const unsigned char* keep;  // [n] individuals in the analysis (nullptr = all)
+  int n_stat;                 // individuals in the analysis (n when keep is null)
   int t, ldt;                 // traits, stride of per-trait arrays
   int ldc;                    // row stride of the contraction output (>= t + l)
   int G;                      // missing-value patterns

@@ -623,10 +623,18 @@ static int pinned_variant(int prec) {
   return -1;
 }
 
+// Shape classes of the tuning cache: M in three bands, N rounded up to a
+// power of two (the request server changes N whenever its batch changes).
+static int m_class(int M) { return M >= 65536 ? 2 : (M >= 4096 ? 1 : 0); }
+static int n_class(int N) {
+  int c = 64;
+  while (c < N) c *= 2;
+  return c;
+}
+
 const char* gemm_variant_name(int prec, int M, int N, int K) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
-  const int mclass = M >= 65536 ? 2 : (M >= 4096 ? 1 : 0);
-  auto it = g_tuned.find(TuneKey{prec, mclass, N, K});
+  auto it = g_tuned.find(TuneKey{prec, m_class(M), n_class(N), K});
   return it == g_tuned.end() ? "untuned" : kVariants[it->second].name;
 }
 
@@ -636,8 +644,7 @@ void launch_gemm(int prec, const void* Ahi, const void* Alo, const void* Bhi, co
                  void* C, int M, int N, int K, int lda, int ldb, int ldc, hipStream_t st) {
   int v = pinned_variant(prec);
   if (v < 0) {
-    const int mclass = M >= 65536 ? 2 : (M >= 4096 ? 1 : 0);
-    const TuneKey key{prec, mclass, N, K};
+    const TuneKey key{prec, m_class(M), n_class(N), K};
     {
       std::lock_guard<std::mutex> lk(g_tune_mu);
       auto it = g_tuned.find(key);

@@ -36,6 +36,8 @@ void launch_epilogue_pst(const DevScan& d, int prec, int lb, hipStream_t st);
 int patsnp_width(int r, int lb);
 void launch_fallback(const DevScan& d, int lb, hipStream_t st);
 void launch_reset_counter(unsigned long long* c, int idx, hipStream_t st);
+// r = 1: out[s] = sum over the subset's rows (d.keep) of x^2
+void launch_subset_sbr(const DevScan& d, double* out, hipStream_t st);
 
 // contraction (oagpu_gemm.hip): C[c][j] = sum_k A[c][k] B[j][k]
 // A: M rows of K, row stride lda (padded to gemm_row_pad() rows), B: N rows

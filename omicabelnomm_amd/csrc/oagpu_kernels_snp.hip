@@ -120,7 +120,7 @@ __global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const double xv = x[v][e];
-          if (k0 + e >= d.n) {
+          if (k0 + e >= d.n || (d.keep && !d.keep[k0 + e])) {
             x[v][e] = 0.0;
           } else if (xv == xv) {
             if (!impute) sum[v] += xv;
@@ -164,7 +164,7 @@ __global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo) {
   for (int v = 0; v < NV; ++v) {
     const double sc = block_sum256(sum[v], shd);
     const double nc = block_sum256(cnt[v], shd);
-    mean[v] = sc / ((double)d.n - nc);
+    mean[v] = sc / ((double)d.n_stat - nc);
     if (live[v / R]) missing += nc;
   }
   if (missing > 0.0) run(true);   // block-uniform; each thread rewrites its own rows
@@ -386,9 +386,9 @@ template <int R, int LB>
 __device__ void stats_emit(const DevScan& d, int y, int s, const double* b, double rss) {
   const int l = d.l, p = d.p;
   const int ymiss = d.ymiss[y];
-  const int n_c = d.n - ymiss - p;
-  const double vf = (double)(n_c + p) / (double)d.n;
-  const double R2 = 1.0 - ((rss / (double)(n_c - 1)) / (d.sst[y] / (double)(d.n - 1)));
+  const int n_c = d.n_stat - ymiss - p;
+  const double vf = (double)(n_c + p) / (double)d.n_stat;
+  const double R2 = 1.0 - ((rss / (double)(n_c - 1)) / (d.sst[y] / (double)(d.n_stat - 1)));
   const double* vx = d.varx + (size_t)s * p;
   const size_t pair = (size_t)y * d.m_total + (size_t)(d.s0 + s);
   const bool dense = d.d_beta != nullptr;
@@ -465,7 +465,7 @@ __device__ void stats_emit(const DevScan& d, int y, int s, const double* b, doub
     hr.trait = d.trait_offset + y;
     hr.snp = d.snp_offset + d.s0 + s;
     hr.h = h;
-    hr.n_used = d.n - ymiss;
+    hr.n_used = d.n_stat - ymiss;
     hr.b = b[j];
     hr.se = se;
     hr.t = tv;
@@ -485,6 +485,7 @@ __device__ double explicit_rss_thread(const DevScan& d, int y, int s, const doub
   for (int c = 0; c < R; ++c) mean[c] = d.col_mean[(size_t)s * R + c];
   double acc = 0.0;
   for (int k = 0; k < d.n; ++k) {
+    if (d.keep && !d.keep[k]) continue;
     double yv = load_elem(d.Y, d.ydt, (size_t)y * d.ldy + k);
     if (yv != yv) yv = 0.0;
     if (yv == 0.0) continue;
@@ -998,6 +999,7 @@ __global__ __launch_bounds__(256) void k_fallback(DevScan d) {
     for (int c = 0; c < R; ++c) mean[c] = d.col_mean[(size_t)s * R + c];
     double acc = 0.0;
     for (int k = lane; k < d.n; k += 64) {
+      if (d.keep && !d.keep[k]) continue;
       double yv = load_elem(d.Y, d.ydt, (size_t)y * d.ldy + k);
       if (yv != yv) yv = 0.0;
       if (yv == 0.0) continue;
@@ -1020,6 +1022,30 @@ __global__ __launch_bounds__(256) void k_fallback(DevScan d) {
 }
 
 __global__ void k_reset_counter(unsigned long long* c, int idx) { c[idx] = 0ull; }
+
+// Sbr = x'x over the subset's rows for r = 1 (a request that shares the
+// chunk's operand but not its individuals); x is imputed with the chunk's
+// column mean.  One block per SNP.
+__global__ __launch_bounds__(256) void k_subset_sbr(DevScan d, double* out) {
+  __shared__ double shd[4];
+  const int s = blockIdx.x;
+  const int gs = d.s0 + s;
+  const int src = d.col_src ? d.col_src[gs] : gs;
+  const double mean = d.col_mean[s];
+  double acc = 0.0;
+  for (int k0 = 4 * threadIdx.x; k0 < d.n; k0 += 1024) {
+    double x[4];
+    geno_value4(d, src, -1, k0, x);
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      if (k0 + e < d.n && d.keep[k0 + e]) {
+        const double v = (x[e] == x[e]) ? x[e] : mean;
+        acc += v * v;
+      }
+  }
+  acc = block_sum256(acc, shd);
+  if (threadIdx.x == 0) out[s] = acc;
+}
 
 // ---------------------------------------------------------------------------
 // launchers
@@ -1104,6 +1130,10 @@ void launch_fallback(const DevScan& d, int lb, hipStream_t st) {
 #define OA_FB(R_, LB_) hipLaunchKernelGGL((k_fallback<R_, LB_>), grid, block, 0, st, d);
   OA_DISPATCH_RLB(0, 0, OA_FB)
 #undef OA_FB
+}
+
+void launch_subset_sbr(const DevScan& d, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_subset_sbr, dim3(d.mc), dim3(256), 0, st, d, out);
 }
 
 void launch_reset_counter(unsigned long long* c, int idx, hipStream_t st) {

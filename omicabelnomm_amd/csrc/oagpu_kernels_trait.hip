@@ -34,7 +34,7 @@ __global__ __launch_bounds__(256) void k_ystats(DevScan d, void* Bhi, void* Blo,
     const int k = k0 + tid;
     double v = 0.0;
     bool miss = false;
-    if (k < d.n) {
+    if (k < d.n && (!d.keep || d.keep[k])) {   // rows outside the subset stay 0
       v = load_elem(d.Y, d.ydt, ybase + k);
       miss = (v != v);
       if (miss) v = 0.0;
@@ -80,7 +80,7 @@ __global__ __launch_bounds__(256) void k_ystats(DevScan d, void* Bhi, void* Blo,
     yzero[y] = (int)nzero;
     yy[y] = sq;
     // sumSquares: sxx - sx^2 / (rows - nans)
-    sst[y] = sq - (sum * sum) / (double)(d.n - written);
+    sst[y] = sq - (sum * sum) / (double)(d.n_stat - written);
     hash[2 * y] = (unsigned long long)H1;
     hash[2 * y + 1] = (unsigned long long)H2;
   }
@@ -88,6 +88,7 @@ __global__ __launch_bounds__(256) void k_ystats(DevScan d, void* Bhi, void* Blo,
   for (int j = 0; j < d.l; ++j) {
     double acc = 0.0;
     for (int k = tid; k < d.n; k += 256) {
+      if (d.keep && !d.keep[k]) continue;
       double v = load_elem(d.Y, d.ydt, ybase + k);
       if (v == v) acc += d.XL_cm[(size_t)j * d.n_pad + k] * v;
     }

@@ -66,9 +66,29 @@ class Stats(ctypes.Structure):
     ]
 
 
-EXPORTS = ["oagpu_create", "oagpu_destroy", "oagpu_last_error", "oagpu_set_covariates",
+class ServerParams(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("hbm_budget", ctypes.c_longlong),
+                ("max_active", ctypes.c_int), ("chunk_snps", ctypes.c_int)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [
+        ("t", ctypes.c_int), ("l", ctypes.c_int),
+        ("Y", ctypes.c_void_p), ("ldy", ctypes.c_longlong),
+        ("XL", ctypes.c_void_p), ("ldxl", ctypes.c_longlong),
+        ("keep", ctypes.c_void_p),
+        ("min_p_store", ctypes.c_float), ("min_p_disp", ctypes.c_float), ("min_r2_store", ctypes.c_float),
+        ("disp_cov", ctypes.c_int), ("store_all", ctypes.c_int),
+        ("max_hits", ctypes.c_longlong),
+    ]
+
+
+EXPORTS = ["oagpu_create", "oagpu_destroy", "oagpu_last_error", "oagpu_set_subset", "oagpu_set_covariates",
            "oagpu_set_interactions", "oagpu_load_traits", "oagpu_load_genotypes", "oagpu_scan",
-           "oagpu_last_timings", "oagpu_gemm_variant", "oagpu_tvalue"]
+           "oagpu_scan_async", "oagpu_scan_wait", "oagpu_host_alloc", "oagpu_host_free",
+           "oagpu_last_timings", "oagpu_gemm_variant", "oagpu_tvalue",
+           "oagpu_panel_create", "oagpu_panel_destroy", "oagpu_server_create", "oagpu_server_destroy",
+           "oagpu_server_last_error", "oagpu_server_submit", "oagpu_server_poll", "oagpu_server_wait"]
 
 _lib = None
 
@@ -88,6 +108,7 @@ def load():
     L.oagpu_destroy.restype = None
     L.oagpu_last_error.argtypes = [vp]
     L.oagpu_last_error.restype = ctypes.c_char_p
+    L.oagpu_set_subset.argtypes = [vp, vp]
     L.oagpu_set_covariates.argtypes = [vp, vp, ll]
     L.oagpu_set_interactions.argtypes = [vp, vp, ll, i]
     L.oagpu_load_traits.argtypes = [vp, vp, i, i, ll, i, i]
@@ -98,8 +119,21 @@ def load():
     L.oagpu_gemm_variant.restype = ctypes.c_char_p
     L.oagpu_tvalue.argtypes = [ctypes.c_float]
     L.oagpu_tvalue.restype = ctypes.c_double
-    for f in ["oagpu_set_covariates", "oagpu_set_interactions", "oagpu_load_traits",
-              "oagpu_load_genotypes", "oagpu_scan", "oagpu_last_timings"]:
+    L.oagpu_panel_create.argtypes = [i, i, ll, vp, i, i, ll, ctypes.POINTER(vp)]
+    L.oagpu_panel_destroy.argtypes = [vp]
+    L.oagpu_panel_destroy.restype = None
+    L.oagpu_server_create.argtypes = [vp, ctypes.POINTER(ServerParams), ctypes.POINTER(vp)]
+    L.oagpu_server_destroy.argtypes = [vp]
+    L.oagpu_server_destroy.restype = None
+    L.oagpu_server_last_error.argtypes = [vp]
+    L.oagpu_server_last_error.restype = ctypes.c_char_p
+    L.oagpu_server_submit.argtypes = [vp, ctypes.POINTER(Request), ctypes.POINTER(ll)]
+    L.oagpu_server_poll.argtypes = [vp, ll]
+    L.oagpu_server_wait.argtypes = [vp, ll, vp, ll, ctypes.POINTER(Stats), ctypes.POINTER(ctypes.c_double),
+                                    ctypes.c_char_p, i]
+    for f in ["oagpu_set_subset", "oagpu_set_covariates", "oagpu_set_interactions", "oagpu_load_traits",
+              "oagpu_load_genotypes", "oagpu_scan", "oagpu_last_timings", "oagpu_panel_create",
+              "oagpu_server_create", "oagpu_server_submit", "oagpu_server_poll", "oagpu_server_wait"]:
         getattr(L, f).restype = i
     _lib = L
     return L

@@ -253,6 +253,16 @@ class Scanner:
         except Exception:
             pass
 
+    def set_subset(self, keep):
+        """Analyse only the individuals with keep[k] true (before set_covariates)."""
+        if keep is None:
+            self._check(self._L.oagpu_set_subset(self._ctx, None))
+            return
+        k = np.ascontiguousarray(keep, dtype=np.uint8)
+        assert k.shape == (self.n,)
+        self._keep["subset"] = k
+        self._check(self._L.oagpu_set_subset(self._ctx, k.ctypes.data))
+
     def set_covariates(self, XL):
         XL = np.asfortranarray(XL, dtype=np.float64)
         assert XL.shape == (self.n, self.l)

@@ -1,0 +1,150 @@
+"""Individual subsets and the request server against the oracle and the
+plain scan.
+
+- a subset scan (oagpu_set_subset) equals the oracle run on the subset's
+  rows alone;
+- every request the server answers equals the same scan run on its own
+  context, for requests with shared and independent missing patterns and
+  with subsets, over ragged chunks;
+- a request's answer is the same alone and in a mix of concurrent requests
+  (bit-identical in FP64; split-BF16 within 1e-4 SE on beta, as the mixed
+  path is held to against FP64);
+- a budget that admits one request at a time still answers every request.
+"""
+import numpy as np
+import pytest
+import torch
+
+from omicabelnomm_amd import Scanner, synth
+from omicabelnomm_amd.serve import Panel, Server
+from parity_util import compare_fp64, run_oracle
+
+pytestmark = pytest.mark.gpu
+
+THR = dict(min_p_store=0.2, min_p_disp=0.05, min_r2_store=-10.0)
+
+
+def hit_key(h):
+    return np.stack([h["trait"], h["snp"], h["h"], h["n_used"]], axis=1)
+
+
+def assert_hits_equal(a, b, tol):
+    """Same hit set; |db| <= tol*SE, SE and t within tol relative, R2 within
+    tol, -log10 p within 1000*tol (1e-9 at least)."""
+    assert len(a) == len(b), (len(a), len(b))
+    np.testing.assert_array_equal(hit_key(a), hit_key(b))
+    assert np.all(np.abs(a["b"] - b["b"]) <= tol * b["se"]), np.max(np.abs(a["b"] - b["b"]) / b["se"])
+    np.testing.assert_allclose(a["se"], b["se"], rtol=tol, atol=0)
+    np.testing.assert_allclose(a["t"], b["t"], rtol=tol, atol=tol)
+    np.testing.assert_allclose(a["r2"], b["r2"], rtol=0, atol=tol)
+    np.testing.assert_allclose(-np.log10(a["p"]), -np.log10(b["p"]), rtol=0, atol=max(tol * 1e3, 1e-9))
+
+
+def panel_data(n=520, m=230, seed=41):
+    d = synth.make(n=n, l=2, m=m, t=2, miss="none", seed=seed)
+    return d.G.contiguous()   # (m, n) float32
+
+
+def request_data(n, t, l, miss, seed, rate=0.06):
+    rng = np.random.default_rng(1000 + seed)
+    XL = np.column_stack([np.ones(n), rng.standard_normal((n, l - 1))])
+    Y = rng.standard_normal((n, t))
+    if miss == "mcar":
+        Y[rng.random((n, t)) < rate] = np.nan
+    elif miss == "groups":
+        for q in range(3):
+            rows = rng.choice(n, size=int(rate * n), replace=False)
+            Y[np.ix_(rows, np.arange(q, t, 3))] = np.nan
+    return Y, XL
+
+
+def scan_alone(G, Y, XL, keep, precision, **thr):
+    """The same analysis on a context of its own (the plain scan path)."""
+    n, t = Y.shape
+    with Scanner(n, XL.shape[1], 1, precision, max_traits=t, max_hits=1 << 20, store_all=True, **thr) as sc:
+        if keep is not None:
+            sc.set_subset(keep)
+        sc.set_covariates(XL)
+        sc.load_traits(Y)
+        sc.load_genotypes(G.to("cuda"))
+        return sc.scan().hits
+
+
+def test_subset_scan_matches_oracle_on_the_subset(oracle):
+    data = synth.make(n=420, l=4, m=35, t=28, miss="mcar", miss_rate=0.08, planted=6, seed=12)
+    rng = np.random.default_rng(5)
+    keep = rng.random(data.n) < 0.7
+    print("case subset scan", int(keep.sum()), flush=True)
+    with Scanner(data.n, data.l, 1, "fp64", max_traits=data.t, store_all=True, disp_cov=True) as sc:
+        sc.set_subset(keep)
+        sc.set_covariates(data.XL)
+        sc.load_traits(data.Y.cpu().numpy().T)
+        sc.load_genotypes(data.G.cpu().numpy().T)
+        g = sc.scan(dense=True)
+    XR = data.xr_matrix()[keep]
+    o = oracle.scan(data.XL[keep], XR, data.y_matrix()[keep], 1, store_all=True, disp_cov=True, threads=4)
+    compare_fp64(g, o)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+def test_server_answers_equal_plain_scans(precision):
+    G = panel_data()
+    n = G.shape[1]
+    rng = np.random.default_rng(7)
+    keep = rng.random(n) < 0.8
+    reqs = [request_data(n, 30, 3, "groups", 1), request_data(n, 9, 5, "mcar", 2),
+            request_data(n, 17, 2, "none", 3), request_data(n, 4, 6, "mcar", 4)]
+    keeps = [None, None, keep, keep]
+    print("case server", precision, flush=True)
+    panel = Panel(G.to("cuda"))
+    with Server(panel, precision, chunk_snps=64) as srv:   # 4 chunks, the last ragged
+        ids = [srv.submit(Y, XL, k, store_all=True, **THR) for (Y, XL), k in zip(reqs, keeps)]
+        answers = [srv.wait(i) for i in ids]
+    for (Y, XL), k, a in zip(reqs, keeps, answers):
+        want = scan_alone(G, Y, XL, k, precision, **THR)
+        assert_hits_equal(a.hits, want, 1e-12 if precision == "fp64" else 1e-4)
+        assert a.stats["n_chunks"] == 4 and a.latency_ms > 0
+    panel.close()
+
+
+# split-BF16: the batch changes the contraction's shape, so the autotuner may
+# pick another kernel (another summation order); the bound is the one the
+# mixed path is held to against FP64
+@pytest.mark.parametrize("precision,rtol", [("fp64", 0.0), ("bf16x3", 1e-4)])
+def test_request_alone_and_in_a_mix(precision, rtol):
+    G = panel_data(seed=43)
+    n = G.shape[1]
+    target = request_data(n, 21, 4, "mcar", 11)
+    others = [request_data(n, 40, 3, "groups", 12), request_data(n, 5, 2, "none", 13),
+              request_data(n, 64, 7, "mcar", 14)]
+    print("case alone/mix", precision, flush=True)
+    panel = Panel(G.to("cuda"))
+    with Server(panel, precision, chunk_snps=48) as srv:
+        alone = srv.wait(srv.submit(*target, store_all=True, **THR))
+        ids = [srv.submit(*others[0], store_all=True, **THR), srv.submit(*others[1], store_all=True, **THR)]
+        tid = srv.submit(*target, store_all=True, **THR)
+        ids.append(srv.submit(*others[2], store_all=True, **THR))
+        mixed = srv.wait(tid)
+        for i in ids:
+            srv.wait(i)
+    if rtol == 0.0:
+        assert alone.hits.tobytes() == mixed.hits.tobytes()
+    else:
+        assert_hits_equal(alone.hits, mixed.hits, rtol)
+    panel.close()
+
+
+def test_budget_admits_one_at_a_time():
+    G = panel_data(n=300, m=100, seed=44)
+    n = G.shape[1]
+    reqs = [request_data(n, 6 + 3 * i, 2 + i, "mcar", 20 + i) for i in range(3)]
+    print("case budget", flush=True)
+    panel = Panel(G.to("cuda"))
+    with Server(panel, "fp64", hbm_budget=1, chunk_snps=32) as srv:
+        ids = [srv.submit(Y, XL, store_all=True, **THR) for Y, XL in reqs]
+        answers = [srv.wait(i) for i in ids]
+    for (Y, XL), a in zip(reqs, answers):
+        assert_hits_equal(a.hits, scan_alone(G, Y, XL, None, "fp64", **THR), 1e-12)
+    # one at a time: each finished before the next started, so latencies grow
+    assert answers[0].latency_ms < answers[2].latency_ms
+    panel.close()
