@@ -38,7 +38,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--config", type=int, default=2,
+                    help="0-2: one scan per step; 3: 64 batched uneven jobs; 4: Poisson request stream")
+    ap.add_argument("--jobs", type=int, default=0, help="requests of configs 3/4 (0 = the config's count)")
+    ap.add_argument("--chunk", type=int, default=8192, help="server chunk (SNPs), configs 3/4")
+    ap.add_argument("--budget-gib", type=float, default=160.0, help="server HBM budget, configs 3/4")
+    ap.add_argument("--fp64-jobs", type=int, default=-1,
+                    help="requests of configs 3/4 rerun one at a time in FP64 (-1 = all)")
     ap.add_argument("--precision", default="bf16x3", choices=["fp64", "fp32", "bf16x3"])
     ap.add_argument("--fp64-steps", type=int, default=1, help="timed FP64 steps for the ratio (0 = skip)")
     ap.add_argument("--m", type=int, default=0, help="override SNP count (testing only)")
@@ -126,6 +132,132 @@ def cpu_baseline(data, budget_s):
                       % (ms, data.t), "seconds": round(dt, 3)}
 
 
+def percentile(xs, q):
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if xs else None
+
+
+def bench_serve(args):
+    """Configs 3 and 4: a float16 panel resident in HBM and requests answered
+    by the server (continuous batching).  value = tests/s with every request
+    submitted at once (the server's capacity); for config 4 the same
+    requests are then replayed as a Poisson stream for the latency figures;
+    fp64 = the same requests run one at a time as plain FP64 scans."""
+    import threading
+    import torch
+    from omicabelnomm_amd import Scanner, synth
+    from omicabelnomm_amd.serve import Panel, Server
+    cfg = synth.SERVE_CONFIGS[args.config]
+    n = cfg["n"]
+    m = args.m or cfg["m"]
+    thr = dict(min_p_store=args.p_store, min_p_disp=args.p_store, min_r2_store=-10.0)
+    specs = synth.job_specs(args.config, jobs=args.jobs or None)
+    t0 = time.perf_counter()
+    G = synth.panel(args.config, m=m)
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t0
+    arrays = [sp.arrays(n) for sp in specs]
+    tests = sum(sp.t for sp in specs) * m
+    panel = Panel(G)
+
+    def serve(arrivals):
+        srv = Server(panel, args.precision, hbm_budget=int(args.budget_gib * 2**30), max_active=64,
+                     chunk_snps=args.chunk)
+        warm = synth.JobSpec(t=64, l=3, keep=None, miss_rate=0.02, arrival=0.0, seed=1)
+        srv.wait(srv.submit(*warm.arrays(n), **thr))   # autotuner, code objects
+        ids = [None] * len(specs)
+        start = time.perf_counter()
+
+        def submitter():
+            for i, sp in enumerate(specs):
+                delay = start + arrivals[i] - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                ids[i] = srv.submit(arrays[i][0], arrays[i][1], sp.keep, max_hits=1 << 22, **thr)
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        th.join()
+        answers = [srv.wait(i) for i in ids]
+        wall = time.perf_counter() - start
+        srv.close()
+        return answers, wall
+
+    answers, wall = serve([0.0] * len(specs))
+    value = tests / wall
+    lat = [a.latency_ms for a in answers]
+    print("served %d requests at once in %.2f s" % (len(specs), wall), file=sys.stderr, flush=True)
+    stream = None
+    if cfg["rate"]:
+        # the same requests as a Poisson stream, at the configured rate or 80 %
+        # of the measured capacity, whichever is lower
+        rate = min(cfg["rate"], 0.8 * len(specs) / wall)
+        scale = cfg["rate"] / rate
+        s_answers, s_wall = serve([sp.arrival * scale for sp in specs])
+        s_lat = [a.latency_ms for a in s_answers]
+        stream = {"rate_per_s": rate, "seconds": s_wall, "tests_per_s": tests / s_wall,
+                  "p50_ms": percentile(s_lat, 50), "p99_ms": percentile(s_lat, 99), "max_ms": max(s_lat)}
+        print("stream at %.2f/s: p50 %.0f ms p99 %.0f ms" % (rate, stream["p50_ms"], stream["p99_ms"]),
+              file=sys.stderr, flush=True)
+
+    # the same requests one at a time, plain FP64 scans of the panel
+    fp64 = prec = None
+    k = len(specs) if args.fp64_jobs < 0 else min(args.fp64_jobs, len(specs))
+    if k > 0:
+        dt64, tests64, dmax, same, nh = 0.0, 0, 0.0, True, 0
+        for sp, (Y, XL), a in zip(specs[:k], arrays[:k], answers[:k]):
+            with Scanner(n, sp.l, 1, "fp64", max_traits=sp.t, max_hits=1 << 22, **thr) as sc:
+                if sp.keep is not None:
+                    sc.set_subset(sp.keep)
+                sc.set_covariates(XL)
+                sc.load_traits(Y)
+                sc.load_genotypes(G)
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                res = sc.scan()
+                dt64 += time.perf_counter() - t1
+            tests64 += sp.t * m
+            keys = lambda h: list(zip(h["trait"].tolist(), h["snp"].tolist(), h["h"].tolist()))
+            strong = lambda h: set(keys(h[h["p"] < 1e-8]))
+            same = same and strong(a.hits) == strong(res.hits)
+            where = {kk: i for i, kk in enumerate(keys(res.hits))}
+            for j, kk in enumerate(keys(a.hits)):
+                i = where.get(kk)
+                if i is not None and a.hits["p"][j] > 0 and res.hits["p"][i] > 0:
+                    dmax = max(dmax, abs(np.log10(a.hits["p"][j]) - np.log10(res.hits["p"][i])))
+            nh += int((res.hits["p"] < 1e-8).sum())
+        print("fp64 one at a time: %d requests in %.2f s" % (k, dt64), file=sys.stderr, flush=True)
+        fp64 = {"value": tests64 / dt64, "mode": "one request at a time, plain FP64 scan of the resident panel",
+                "requests": k, "seconds": round(dt64, 3)}
+        prec = {"vs": "fp64 one-at-a-time answers of %d requests (hits stored at p<%g)" % (k, args.p_store),
+                "hits_p_lt_1e-8_identical": same, "n_hits_p_lt_1e-8": nh, "max_abs_dlog10p": dmax}
+    panel.close()
+    widths = [sp.t for sp in specs]
+    line = {
+        "metric": METRIC,
+        "value": value, "unit": "SNPxtrait tests/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"bf16x3": "bf16x3(fp32 acc)+f64", "fp32": "f32+f64", "fp64": "f64"}[args.precision],
+        "data": "synthetic (seeded; float16 dosage panel generated on the GPU, host traits)",
+        "config": {"workload": "C%d: panel n=%d x m=%d float16 resident; %d requests, traits %d..%d (sum %d), "
+                               "%s, MCAR 0-5%%, covariates 2-16 + intercept"
+                               % (args.config, n, m, len(specs), min(widths), max(widths), sum(widths),
+                                  "individual subsets %d-%d" % cfg["subset"] if cfg["subset"] else
+                                  "all individuals"),
+                   "thresholds": {"minPstore": args.p_store, "minPdisp": args.p_store, "minR2store": -10.0},
+                   "precision": args.precision, "chunk_snps": args.chunk, "gen_seconds": round(gen_s, 2),
+                   "value_is": "all requests submitted at once: total tests / time to the last answer"},
+        "latency_ms": {"p50": percentile(lat, 50), "p99": percentile(lat, 99), "max": max(lat),
+                       "note": "submitted together"},
+        "stream": stream,
+        "roofline": None,
+        "cpu_baseline": None,
+        "fp64": fp64,
+        "speedup_vs_fp64": (value / fp64["value"]) if fp64 else None,
+        "precision_check": prec,
+    }
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -138,6 +270,9 @@ def main():
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.config in (3, 4):
+        bench_serve(args)
+        return
     if args.copy_probe:
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)

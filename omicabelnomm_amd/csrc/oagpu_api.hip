@@ -130,7 +130,7 @@ DevScan oa::make_dev(const oagpu_ctx* c) {
   d.UA = c->UA; d.UA_invd = c->UA_invd; d.pat_pd = c->pat_pd;
   d.nU = c->compact_corr ? c->nU : 0;
   d.uidx = c->uidx; d.XLu = c->XLu; d.pu_off = c->pu_off; d.pu_list = c->pu_list;
-  d.Xu = c->Xu; d.ldxu = c->ldxu; d.pst = c->pst;
+  d.Xu = c->Xu; d.xu_dt = DT_F64; d.ldxu = c->ldxu; d.pst = c->pst;
   d.Gm = c->Gm; d.gdt = c->gdt; d.ldg = c->ldg; d.col_src = c->col_src; d.col_env = c->col_env;
   d.g_vec = c->Gm && ((uintptr_t)c->Gm % 16 == 0) && (c->ldg % 4 == 0);
   d.col_mean = c->col_mean; d.Sbr = c->Sbr; d.Str = c->Str; d.varx = c->varx; d.Ccorr = c->Ccorr;
@@ -533,8 +533,8 @@ static int build_union(oagpu_ctx* c, const std::vector<int>& pat_rep, const std:
     if (uidx[k] >= 0)
       for (int j = 0; j < l; ++j) XLu[(size_t)uidx[k] * l + j] = c->XL_rm_h[(size_t)k * l + j];
   HIPCK(c, grow((void**)&c->XLu, &c->XLu_cap, sizeof(double) * std::max<size_t>(XLu.size(), 1)));
-  HIPCK(c, hipMemcpyAsync(c->uidx, uidx.data(), sizeof(int) * uidx.size(), hipMemcpyHostToDevice, c->st));
   HIPCK(c, hipMemcpyAsync(c->XLu, XLu.data(), sizeof(double) * XLu.size(), hipMemcpyHostToDevice, c->st));
+  HIPCK(c, hipMemcpyAsync(c->uidx, uidx.data(), sizeof(int) * uidx.size(), hipMemcpyHostToDevice, c->st));
   launch_union_list(n, G, c->rows, c->pat_rep, c->pat_cnt, c->pu_off, c->uidx, c->pu_list, c->st);
   HIPCK(c, hipGetLastError());
   HIPCK(c, hipStreamSynchronize(c->st));   // the host vectors above are released on return

@@ -86,8 +86,10 @@ struct DevScan {
   double* Sbr;           // [mc][r(r+1)/2] packed upper
   double* Str;           // [mc][l][r]
   double* varx;          // [mc][p]
-  double* Ccorr;         // [mc][Q][G]
-  double* Xu;            // [nU][ldxu] imputed genotype values at the rows of U
+  double* Ccorr;         // [G][Q][mc]: pattern, correction term, SNP
+  void* Xu;              // [nU][ldxu] imputed genotype values at the rows of U
+  int xu_dt;             // element type of Xu: DT_F64, or the panel's DT_F32 / DT_F16
+                         // when its values are exact there (no missing genotypes)
   double* pst;           // [mc][G][patsnp_width] per-(SNP, pattern) solve pieces
   long long ldxu;        // >= mc*r
   const void* Cbuf;      // [mc*r][ldc]

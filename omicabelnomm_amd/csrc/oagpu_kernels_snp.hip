@@ -137,10 +137,14 @@ __global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo) {
 #pragma unroll
         for (int e = 0; e < 4; ++e)
           if (uu[e] >= 0) {
-            double* dst = d.Xu + (size_t)uu[e] * d.ldxu + (size_t)sb * R;
+            const size_t o = (size_t)uu[e] * d.ldxu + (size_t)sb * R;
 #pragma unroll
             for (int v = 0; v < NV; ++v)
-              if (live[v / R]) dst[v] = x[v][e];
+              if (live[v / R]) {
+                if (d.xu_dt == DT_F64) static_cast<double*>(d.Xu)[o + v] = x[v][e];
+                else if (d.xu_dt == DT_F32) static_cast<float*>(d.Xu)[o + v] = (float)x[v][e];
+                else static_cast<_Float16*>(d.Xu)[o + v] = (_Float16)(float)x[v][e];
+              }
           }
       }
 #pragma unroll
@@ -312,6 +316,10 @@ __global__ void k_snp_gram(DevScan d) {
 }
 
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ double widen(double v) { return v; }
+__device__ __forceinline__ double widen(float v) { return (double)v; }
+__device__ __forceinline__ double widen(_Float16 v) { return (double)(float)v; }
+
 // generate_correction_missings_Str (src/Algorithm.cpp:1260-1315) for every
 // pattern: C_lr = sum over the pattern's missing rows of xl x', C_rr of x x'.
 // The genotype values come from the compact u-major rows of U written by the
@@ -319,7 +327,7 @@ __global__ void k_snp_gram(DevScan d) {
 // positions and covariate rows are wave-uniform, and the 64 SNPs' values at
 // one row are contiguous.  Blocks are numbered so that the pattern groups of
 // one 64-SNP tile run back to back on one XCD and share its L2.
-template <int R, int LB>
+template <int R, int LB, typename XT>
 __global__ __launch_bounds__(256) void k_corr(DevScan d, int gq_count) {
   const int lane = threadIdx.x & 63;
   const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
@@ -341,10 +349,10 @@ __global__ __launch_bounds__(256) void k_corr(DevScan d, int gq_count) {
 #pragma unroll 4
   for (int i = beg; i < end; ++i) {
     const int u = d.pu_list[i];
-    const double* xr = d.Xu + (size_t)u * d.ldxu + (size_t)se * R;
+    const XT* xr = static_cast<const XT*>(d.Xu) + (size_t)u * d.ldxu + (size_t)se * R;
     double x[R];
 #pragma unroll
-    for (int c = 0; c < R; ++c) x[c] = xr[c];
+    for (int c = 0; c < R; ++c) x[c] = widen(xr[c]);
     const double* xl = d.XLu + (size_t)u * l;
 #pragma unroll
     for (int j = 0; j < LB; ++j)
@@ -359,14 +367,13 @@ __global__ __launch_bounds__(256) void k_corr(DevScan d, int gq_count) {
       for (int a = 0; a <= b; ++a) rr[upk(a, b)] += x[a] * x[b];
   }
   if (s < d.mc) {
-    const size_t base = (size_t)s * d.Q;
 #pragma unroll
     for (int j = 0; j < LB; ++j)
       if (j < l)
 #pragma unroll
-        for (int c = 0; c < R; ++c) d.Ccorr[(base + j * R + c) * d.G + g] = lr[j][c];
+        for (int c = 0; c < R; ++c) d.Ccorr[((size_t)g * d.Q + j * R + c) * d.mc + s] = lr[j][c];
 #pragma unroll
-    for (int e = 0; e < RR; ++e) d.Ccorr[(base + l * R + e) * d.G + g] = rr[e];
+    for (int e = 0; e < RR; ++e) d.Ccorr[((size_t)g * d.Q + l * R + e) * d.mc + s] = rr[e];
   }
 }
 
@@ -574,14 +581,13 @@ __global__ __launch_bounds__(256) void k_epilogue(DevScan d) {
 #pragma unroll
   for (int e = 0; e < RR; ++e) Crr[e] = 0.0;
   if (cnt > 0) {
-    const size_t base = (size_t)s * d.Q;
 #pragma unroll
     for (int j = 0; j < LB; ++j) {
       if (j < l) {
         double tot = 0.0;
 #pragma unroll
         for (int c = 0; c < R; ++c) {
-          const double v = d.Ccorr[(base + j * R + c) * G + g];
+          const double v = d.Ccorr[((size_t)g * d.Q + j * R + c) * d.mc + s];
           tot += v;
           if (R > 1) Clr[R > 1 ? j : 0][c] = v;
         }
@@ -589,7 +595,7 @@ __global__ __launch_bounds__(256) void k_epilogue(DevScan d) {
       }
     }
 #pragma unroll
-    for (int e = 0; e < RR; ++e) Crr[e] = d.Ccorr[(base + l * R + e) * G + g];
+    for (int e = 0; e < RR; ++e) Crr[e] = d.Ccorr[((size_t)g * d.Q + l * R + e) * d.mc + s];
   }
   // D_q: Sbr corrected only in rows j < min(l, r) (src/Algorithm.cpp:1300).
   double Sc[RR];
@@ -744,7 +750,8 @@ __global__ __launch_bounds__(256) void k_patsnp(DevScan d) {
   const int G = d.G, l = d.l;
   const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
   if (idx >= (long long)d.mc * G) return;
-  const int s = (int)(idx / G), g = (int)(idx % G);
+  // SNP fastest: the corrections of one pattern are contiguous in SNPs
+  const int g = (int)(idx / d.mc), s = (int)(idx % d.mc);
   const int cnt = d.pat_cnt[g];
   double X[LB][R], Clr[LB][R], Crr[RR];
 #pragma unroll
@@ -759,21 +766,20 @@ __global__ __launch_bounds__(256) void k_patsnp(DevScan d) {
   if (cnt > 0) {
     // every column's correction is subtracted from the first SNP column
     // (src/Algorithm.cpp:1390-1396)
-    const size_t base = (size_t)s * d.Q;
 #pragma unroll
     for (int j = 0; j < LB; ++j)
       if (j < l) {
         double tot = 0.0;
 #pragma unroll
         for (int c = 0; c < R; ++c) {
-          const double v = d.Ccorr[(base + j * R + c) * G + g];
+          const double v = d.Ccorr[((size_t)g * d.Q + j * R + c) * d.mc + s];
           tot += v;
           Clr[j][c] = v;
         }
         X[j][0] -= tot;
       }
 #pragma unroll
-    for (int e = 0; e < RR; ++e) Crr[e] = d.Ccorr[(base + l * R + e) * G + g];
+    for (int e = 0; e < RR; ++e) Crr[e] = d.Ccorr[((size_t)g * d.Q + l * R + e) * d.mc + s];
   }
   // D_q: Sbr corrected only in rows a < min(l, r) (src/Algorithm.cpp:1300)
   double Sc[RR];
@@ -822,7 +828,7 @@ __global__ __launch_bounds__(256) void k_patsnp(DevScan d) {
       Sc[upk(j, i)] = aji / ajj;
     }
   }
-  double* out = d.pst + (size_t)idx * T::W;
+  double* out = d.pst + ((size_t)s * G + g) * T::W;
   // M = U_A^-1 X
   double Mv[LB][R];
 #pragma unroll
@@ -1092,7 +1098,10 @@ void launch_snp_gram(const DevScan& d, int prec, int lb, hipStream_t st) {
 void launch_corr(const DevScan& d, int lb, hipStream_t st) {
   const int tiles = (d.mc + 63) / 64, gq = (d.G + 3) / 4;
   dim3 grid((unsigned)(((tiles + 7) / 8) * 8 * gq)), block(256);
-#define OA_CR(R_, LB_) hipLaunchKernelGGL((k_corr<R_, LB_>), grid, block, 0, st, d, gq);
+#define OA_CR(R_, LB_)                                                                         \
+  if (d.xu_dt == DT_F64) hipLaunchKernelGGL((k_corr<R_, LB_, double>), grid, block, 0, st, d, gq); \
+  else if (d.xu_dt == DT_F32) hipLaunchKernelGGL((k_corr<R_, LB_, float>), grid, block, 0, st, d, gq); \
+  else hipLaunchKernelGGL((k_corr<R_, LB_, _Float16>), grid, block, 0, st, d, gq);
   OA_DISPATCH_RLB(0, 0, OA_CR)
 #undef OA_CR
 }

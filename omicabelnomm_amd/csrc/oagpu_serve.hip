@@ -34,7 +34,21 @@ struct oagpu_panel {
   void* G = nullptr;
   long long ldg = 0;
   bool own = false;
+  bool has_nan = true;   // any missing genotype
 };
+
+namespace {
+// flag = 1 if any of the panel's n x m values is NaN
+__global__ void k_any_nan(const void* G, int dt, long long ld, int n, long long m, int* flag) {
+  const long long total = (long long)n * m;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long s = i / n, k = i - s * n;
+    const double v = load_elem(G, dt, (size_t)(s * ld + k));
+    if (v != v) *flag = 1;
+  }
+}
+}  // namespace
 
 namespace {
 
@@ -92,7 +106,11 @@ struct oagpu_server {
   long long ldop = 0;
   int n_pad = 0;
   void *Abuf = nullptr, *Ahi = nullptr, *Alo = nullptr;
-  double *col_mean = nullptr, *Sbr = nullptr, *Xu = nullptr;
+  double *col_mean = nullptr, *Sbr = nullptr;
+  // the chunk's genotype values, row-major, for the missing-row corrections:
+  // in the panel's own type when that holds them exactly (no missing value)
+  void* Xu = nullptr;
+  int xu_dt = DT_F64;
   long long ldxu = 0;
   int* uidx = nullptr;               // identity row numbering for the operand pass
   void* Barena = nullptr;
@@ -137,6 +155,14 @@ extern "C" int oagpu_panel_create(int device, int n, long long m, const void* G,
   } else {
     P->G = const_cast<void*>(G);
   }
+  int* flag = nullptr;
+  int h = 1;
+  if (hipMalloc(&flag, sizeof(int)) == hipSuccess && hipMemset(flag, 0, sizeof(int)) == hipSuccess) {
+    hipLaunchKernelGGL(k_any_nan, dim3(4096), dim3(256), 0, 0, P->G, dtype, ld, n, m, flag);
+    if (hipMemcpy(&h, flag, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) h = 1;
+  }
+  if (flag) (void)hipFree(flag);
+  P->has_nan = h != 0;
   *out = P;
   return OAGPU_OK;
 }
@@ -276,6 +302,7 @@ int run_chunk(oagpu_server* S) {
   dp.Sbr = S->Sbr;
   dp.uidx = S->uidx;
   dp.Xu = S->Xu;
+  dp.xu_dt = S->xu_dt;
   dp.ldxu = S->ldxu;
   launch_prep(dp, prec, S->Ahi, S->Alo, any_missing, S->st);
 
@@ -302,6 +329,7 @@ int run_chunk(oagpu_server* S) {
     d.Cbuf = (const char*)S->Cbuf + (size_t)R->col_off * S->c_size;
     d.ldc = S->ldc;
     d.Xu = S->Xu;
+    d.xu_dt = S->xu_dt;
     d.ldxu = S->ldxu;
     d.m_total = (int)P->m;
     d.snp_offset = 0;
@@ -450,7 +478,8 @@ extern "C" int oagpu_server_create(oagpu_panel* P, const oagpu_server_params* pr
   SRVCK(S, hipMalloc(&S->col_mean, sizeof(double) * S->mc));
   SRVCK(S, hipMalloc(&S->Sbr, sizeof(double) * S->mc));
   S->ldxu = ((long long)S->mc + 7) / 8 * 8;
-  SRVCK(S, hipMalloc(&S->Xu, sizeof(double) * (size_t)P->n * S->ldxu));
+  S->xu_dt = (!P->has_nan && P->dtype != OAGPU_F64) ? (P->dtype == OAGPU_F32 ? DT_F32 : DT_F16) : DT_F64;
+  SRVCK(S, hipMalloc(&S->Xu, dt_bytes(S->xu_dt == DT_F64 ? OAGPU_F64 : P->dtype) * (size_t)P->n * S->ldxu));
   std::vector<int> id(P->n_pad, -1);
   for (int k = 0; k < P->n; ++k) id[k] = k;
   SRVCK(S, hipMalloc(&S->uidx, sizeof(int) * id.size()));

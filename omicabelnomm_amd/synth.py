@@ -167,3 +167,64 @@ def config(idx: int, device="cpu", **over) -> ScanData:
     kw = dict(CONFIGS[idx])
     kw.update(over)
     return make(seed=idx, device=device, name="C%d" % idx, **kw)
+
+
+# Request workloads of the batched-jobs and serving configurations: one
+# float16 dosage panel resident on the device and a stream of scan requests
+# with their own trait count, covariates, individual subset and missingness.
+SERVE_CONFIGS = {
+    # 64 jobs submitted together: t log-uniform [1, 4096], 2-16 covariates
+    # (+ intercept), individual subset of U[2000, 20000] rows, MCAR U[0, 5%]
+    3: dict(n=20_000, m=500_000, jobs=64, t_max=4096, subset=(2000, 20000), rate=None),
+    # Poisson arrivals: t log-uniform [1, 8192], 2-16 covariates, all
+    # individuals, MCAR U[0, 5%]
+    4: dict(n=20_000, m=1_000_000, jobs=32, t_max=8192, subset=None, rate=1.0),
+}
+
+
+@dataclass
+class JobSpec:
+    t: int
+    l: int
+    keep: Optional[np.ndarray]
+    miss_rate: float
+    arrival: float          # seconds after the first request
+    seed: int
+
+    def arrays(self, n):
+        """(Y (n, t), XL (n, l)) float64 host arrays, NaN = missing."""
+        rng = np.random.default_rng(self.seed)
+        XL = np.column_stack([np.ones(n), rng.standard_normal((n, self.l - 1))])
+        Y = rng.standard_normal((n, self.t))
+        if self.miss_rate > 0:
+            Y[rng.random((n, self.t)) < self.miss_rate] = np.nan
+        return Y, XL
+
+
+def job_specs(idx: int, jobs=None, seed=0):
+    cfg = SERVE_CONFIGS[idx]
+    n = cfg["n"]
+    rng = np.random.default_rng(SEED_BASE + 100 * idx + seed)
+    k = jobs or cfg["jobs"]
+    specs, clock = [], 0.0
+    for j in range(k):
+        t = int(round(np.exp(rng.uniform(0.0, np.log(cfg["t_max"])))))
+        l = int(rng.integers(2, 17)) + 1
+        keep = None
+        if cfg["subset"] is not None:
+            size = int(rng.integers(cfg["subset"][0], cfg["subset"][1] + 1))
+            keep = np.zeros(n, dtype=np.uint8)
+            keep[rng.choice(n, size=size, replace=False)] = 1
+        if cfg["rate"]:
+            clock += float(rng.exponential(1.0 / cfg["rate"])) if j else 0.0
+        specs.append(JobSpec(t=t, l=l, keep=keep, miss_rate=float(rng.uniform(0.0, 0.05)), arrival=clock,
+                             seed=SEED_BASE + 1000 * idx + j))
+    return specs
+
+
+def panel(idx: int, device="cuda", m=None, seed=0):
+    """(m, n) float16 dosage panel of a request configuration."""
+    cfg = SERVE_CONFIGS[idx]
+    gen = torch.Generator(device=device)
+    gen.manual_seed(SEED_BASE + 100 * idx + seed)
+    return dosages(m or cfg["m"], cfg["n"], gen, device, dtype=torch.float16)
