@@ -155,7 +155,11 @@ def bench_serve(args):
     G = synth.panel(args.config, m=m)
     torch.cuda.synchronize()
     gen_s = time.perf_counter() - t0
-    arrays = [sp.arrays(n) for sp in specs]
+    arrays = []
+    for i, sp in enumerate(specs):
+        arrays.append(sp.arrays(n))
+        if i % 16 == 15:
+            print("generated %d/%d requests" % (i + 1, len(specs)), file=sys.stderr, flush=True)
     tests = sum(sp.t for sp in specs) * m
     panel = Panel(G)
 
@@ -216,6 +220,7 @@ def bench_serve(args):
                 res = sc.scan()
                 dt64 += time.perf_counter() - t1
             tests64 += sp.t * m
+            print("fp64 request t=%d, %.2f s so far" % (sp.t, dt64), file=sys.stderr, flush=True)
             keys = lambda h: list(zip(h["trait"].tolist(), h["snp"].tolist(), h["h"].tolist()))
             strong = lambda h: set(keys(h[h["p"] < 1e-8]))
             same = same and strong(a.hits) == strong(res.hits)

@@ -345,26 +345,42 @@ __global__ __launch_bounds__(256) void k_corr(DevScan d, int gq_count) {
     for (int c = 0; c < R; ++c) lr[j][c] = 0.0;
 #pragma unroll
   for (int e = 0; e < RR; ++e) rr[e] = 0.0;
+  // The pattern's row positions come in 64 at a time (one coalesced load,
+  // then broadcast lane by lane), so each batch of CORR_BATCH rows issues its
+  // genotype loads back to back instead of one dependent round trip per row.
+  constexpr int CORR_BATCH = 16;
   const int beg = d.pu_off[g], end = d.pu_off[g + 1];
-#pragma unroll 4
-  for (int i = beg; i < end; ++i) {
-    const int u = d.pu_list[i];
-    const XT* xr = static_cast<const XT*>(d.Xu) + (size_t)u * d.ldxu + (size_t)se * R;
-    double x[R];
+  for (int i0 = beg; i0 < end; i0 += 64) {
+    const int nb = __builtin_amdgcn_readfirstlane(min(64, end - i0));
+    const int my_u = lane < nb ? d.pu_list[i0 + lane] : 0;
+    for (int b0 = 0; b0 < nb; b0 += CORR_BATCH) {
+      int uu[CORR_BATCH];
+      double xv[CORR_BATCH][R];
 #pragma unroll
-    for (int c = 0; c < R; ++c) x[c] = widen(xr[c]);
-    const double* xl = d.XLu + (size_t)u * l;
+      for (int q = 0; q < CORR_BATCH; ++q) {
+        uu[q] = __builtin_amdgcn_readlane(my_u, b0 + q);   // lanes >= nb hold row 0: weight 0 below
+        const XT* xr = static_cast<const XT*>(d.Xu) + (size_t)uu[q] * d.ldxu + (size_t)se * R;
 #pragma unroll
-    for (int j = 0; j < LB; ++j)
-      if (j < l) {
-        const double a = xl[j];
-#pragma unroll
-        for (int c = 0; c < R; ++c) lr[j][c] += a * x[c];
+        for (int c = 0; c < R; ++c) xv[q][c] = widen(xr[c]);
       }
 #pragma unroll
-    for (int b = 0; b < R; ++b)
+      for (int q = 0; q < CORR_BATCH; ++q) {
+        if (b0 + q < nb) {   // wave-uniform
+          const double* xl = d.XLu + (size_t)uu[q] * l;
 #pragma unroll
-      for (int a = 0; a <= b; ++a) rr[upk(a, b)] += x[a] * x[b];
+          for (int j = 0; j < LB; ++j)
+            if (j < l) {
+              const double a = xl[j];
+#pragma unroll
+              for (int c = 0; c < R; ++c) lr[j][c] += a * xv[q][c];
+            }
+#pragma unroll
+          for (int b = 0; b < R; ++b)
+#pragma unroll
+            for (int a = 0; a <= b; ++a) rr[upk(a, b)] += xv[q][a] * xv[q][b];
+        }
+      }
+    }
   }
   if (s < d.mc) {
 #pragma unroll
