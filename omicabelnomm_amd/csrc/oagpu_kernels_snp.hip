@@ -348,11 +348,22 @@ __global__ __launch_bounds__(256) void k_corr(DevScan d, int gq_count) {
   // The pattern's row positions come in 64 at a time (one coalesced load,
   // then broadcast lane by lane), so each batch of CORR_BATCH rows issues its
   // genotype loads back to back instead of one dependent round trip per row.
+  // The 64 rows' covariate rows are staged in LDS by the lanes (one row per
+  // lane) and read back as wave-uniform broadcasts.
   constexpr int CORR_BATCH = 16;
+  constexpr int STAGE = LB > 16 ? 32 : 64;   // rows staged at once (LDS: 4 waves x STAGE x LB doubles)
+  __shared__ double xls[4][STAGE][LB];
+  double (*xw)[LB] = xls[threadIdx.x >> 6];
   const int beg = d.pu_off[g], end = d.pu_off[g + 1];
-  for (int i0 = beg; i0 < end; i0 += 64) {
-    const int nb = __builtin_amdgcn_readfirstlane(min(64, end - i0));
+  for (int i0 = beg; i0 < end; i0 += STAGE) {
+    const int nb = __builtin_amdgcn_readfirstlane(min(STAGE, end - i0));
     const int my_u = lane < nb ? d.pu_list[i0 + lane] : 0;
+    if (lane < STAGE) {
+#pragma unroll
+      for (int j = 0; j < LB; ++j)
+        if (j < l) xw[lane][j] = d.XLu[(size_t)my_u * l + j];
+    }
+    __builtin_amdgcn_wave_barrier();
     for (int b0 = 0; b0 < nb; b0 += CORR_BATCH) {
       int uu[CORR_BATCH];
       double xv[CORR_BATCH][R];
@@ -366,7 +377,7 @@ __global__ __launch_bounds__(256) void k_corr(DevScan d, int gq_count) {
 #pragma unroll
       for (int q = 0; q < CORR_BATCH; ++q) {
         if (b0 + q < nb) {   // wave-uniform
-          const double* xl = d.XLu + (size_t)uu[q] * l;
+          const double* xl = xw[b0 + q];
 #pragma unroll
           for (int j = 0; j < LB; ++j)
             if (j < l) {
@@ -381,6 +392,7 @@ __global__ __launch_bounds__(256) void k_corr(DevScan d, int gq_count) {
         }
       }
     }
+    __builtin_amdgcn_wave_barrier();
   }
   if (s < d.mc) {
 #pragma unroll
