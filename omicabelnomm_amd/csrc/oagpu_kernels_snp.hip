@@ -14,6 +14,7 @@
 // corrections add.  In exact arithmetic this is the reference's
 // full-matrix Cholesky solve (same pivots, same failure rule).
 #include "oagpu_kernels.h"
+#include <type_traits>
 
 namespace oa {
 
@@ -84,13 +85,35 @@ __device__ __forceinline__ void store_op4(void* Ahi, void* Alo, size_t o, const 
   }
 }
 
-template <int R, int OP, bool CORR>
+// XT: element type of the compact rows of U (void: no compact rows written).
+__device__ __forceinline__ void store_xu(double* p, double v) { *p = v; }
+__device__ __forceinline__ void store_xu(float* p, double v) { *p = (float)v; }
+__device__ __forceinline__ void store_xu(_Float16* p, double v) { *p = (_Float16)(float)v; }
+__device__ __forceinline__ void store_xu4(double* p, const double* v) {
+  *reinterpret_cast<d4*>(p) = d4{v[0], v[1], v[2], v[3]};
+}
+__device__ __forceinline__ void store_xu4(float* p, const double* v) {
+  *reinterpret_cast<f4*>(p) = f4{(float)v[0], (float)v[1], (float)v[2], (float)v[3]};
+}
+__device__ __forceinline__ void store_xu4(_Float16* p, const double* v) {
+  typedef __attribute__((ext_vector_type(4))) _Float16 h4;
+  *reinterpret_cast<h4*>(p) = h4{(_Float16)(float)v[0], (_Float16)(float)v[1], (_Float16)(float)v[2], (_Float16)(float)v[3]};
+}
+
+template <int R, int OP, typename XT>
 __global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo) {
+  constexpr bool CORR = !std::is_void<XT>::value;
   constexpr int RR = R * (R + 1) / 2;
   constexpr int PREP_SPW = prep_spw<R>();
   constexpr int NV = PREP_SPW * R;
   const int tid = threadIdx.x;
-  const int sb = blockIdx.x * PREP_SPW;
+  // Consecutive SNP blocks run on one XCD (blocks are dealt to the 8 XCDs in
+  // turn), so the 32-byte pieces that neighbouring blocks write into one
+  // compact-row cache line meet in the same L2.
+  const int per = (gridDim.x + 7) >> 3;
+  const int blk = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  const int sb = blk * PREP_SPW;
+  if (sb >= d.mc) return;   // block-uniform, before any barrier
   __shared__ double shd[4];
   int src[NV], env[NV];
   bool live[PREP_SPW];
@@ -113,6 +136,16 @@ __global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo) {
     // two 4-row runs in flight per thread: the loop is latency bound
 #pragma unroll 2
     for (int k0 = 4 * tid; k0 < d.n_pad; k0 += 1024) {
+      // rows in the analysis: below n and inside the subset (keep is padded
+      // to n_pad, so its 4-byte run is in bounds)
+      bool in[4];
+      {
+        const uchar4 kv = d.keep ? *reinterpret_cast<const uchar4*>(d.keep + k0) : make_uchar4(1, 1, 1, 1);
+        in[0] = kv.x && k0 < d.n;
+        in[1] = kv.y && k0 + 1 < d.n;
+        in[2] = kv.z && k0 + 2 < d.n;
+        in[3] = kv.w && k0 + 3 < d.n;
+      }
       double x[NV][4];
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
@@ -120,7 +153,7 @@ __global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const double xv = x[v][e];
-          if (k0 + e >= d.n || (d.keep && !d.keep[k0 + e])) {
+          if (!in[e]) {
             x[v][e] = 0.0;
           } else if (xv == xv) {
             if (!impute) sum[v] += xv;
@@ -130,21 +163,23 @@ __global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo) {
           }
         }
       }
-      if (CORR) {
+      if constexpr (CORR) {
         // rows of U: their imputed values, u-major, for the corrections
         const int4 ui = *reinterpret_cast<const int4*>(d.uidx + k0);
         const int uu[4] = {ui.x, ui.y, ui.z, ui.w};
 #pragma unroll
         for (int e = 0; e < 4; ++e)
           if (uu[e] >= 0) {
-            const size_t o = (size_t)uu[e] * d.ldxu + (size_t)sb * R;
+            XT* dst = static_cast<XT*>(d.Xu) + (size_t)uu[e] * d.ldxu + (size_t)sb * R;
+            if (NV == 4 && live[PREP_SPW - 1]) {
+              // one vector store: sb * R and ldxu are multiples of 4
+              const double v4[4] = {x[0][e], x[NV > 1 ? 1 : 0][e], x[NV > 2 ? 2 : 0][e], x[NV > 3 ? 3 : 0][e]};
+              store_xu4(dst, v4);
+            } else {
 #pragma unroll
-            for (int v = 0; v < NV; ++v)
-              if (live[v / R]) {
-                if (d.xu_dt == DT_F64) static_cast<double*>(d.Xu)[o + v] = x[v][e];
-                else if (d.xu_dt == DT_F32) static_cast<float*>(d.Xu)[o + v] = (float)x[v][e];
-                else static_cast<_Float16*>(d.Xu)[o + v] = (_Float16)(float)x[v][e];
-              }
+              for (int v = 0; v < NV; ++v)
+                if (live[v / R]) store_xu(dst + v, x[v][e]);
+            }
           }
       }
 #pragma unroll
@@ -1083,16 +1118,23 @@ __global__ __launch_bounds__(256) void k_subset_sbr(DevScan d, double* out) {
 
 // ---------------------------------------------------------------------------
 // launchers
+template <int R, typename XT>
+static void prep_rx(const DevScan& d, int prec, void* Ahi, void* Alo, hipStream_t st) {
+  const int nb = (d.mc + prep_spw<R>() - 1) / prep_spw<R>();
+  dim3 grid((nb + 7) / 8 * 8), block(256);   // a multiple of 8 for the XCD mapping
+  if (prec == PREC_FP64) hipLaunchKernelGGL((k_prep<R, PREC_FP64, XT>), grid, block, 0, st, d, Ahi, Alo);
+  else if (prec == PREC_FP32) hipLaunchKernelGGL((k_prep<R, PREC_FP32, XT>), grid, block, 0, st, d, Ahi, Alo);
+  else hipLaunchKernelGGL((k_prep<R, PREC_BF16X3, XT>), grid, block, 0, st, d, Ahi, Alo);
+}
+
+// Narrow compact rows (the panel's own type) only occur with one column per
+// SNP.
 template <int R>
 static void prep_r(const DevScan& d, int prec, void* Ahi, void* Alo, bool corr, hipStream_t st) {
-  dim3 grid((d.mc + prep_spw<R>() - 1) / prep_spw<R>()), block(256);
-#define OA_PR(OP_)                                                                      \
-  if (corr) hipLaunchKernelGGL((k_prep<R, OP_, true>), grid, block, 0, st, d, Ahi, Alo); \
-  else hipLaunchKernelGGL((k_prep<R, OP_, false>), grid, block, 0, st, d, Ahi, Alo);
-  if (prec == PREC_FP64) { OA_PR(PREC_FP64) }
-  else if (prec == PREC_FP32) { OA_PR(PREC_FP32) }
-  else { OA_PR(PREC_BF16X3) }
-#undef OA_PR
+  if (!corr) prep_rx<R, void>(d, prec, Ahi, Alo, st);
+  else if (R > 1 || d.xu_dt == DT_F64) prep_rx<R, double>(d, prec, Ahi, Alo, st);
+  else if (d.xu_dt == DT_F32) prep_rx<1, float>(d, prec, Ahi, Alo, st);
+  else prep_rx<1, _Float16>(d, prec, Ahi, Alo, st);
 }
 
 void launch_prep(const DevScan& d, int prec, void* Ahi, void* Alo, bool corr, hipStream_t st) {
