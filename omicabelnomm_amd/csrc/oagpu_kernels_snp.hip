@@ -452,13 +452,31 @@ __device__ __forceinline__ double pvalue_ref(double t) {
 // Coefficient vectors use a split layout: b[0..LB) covariates (index h < l
 // valid), b[LB..LB+R) the SNP terms (coefficient index h = l + c), so every
 // register index is a compile-time constant.
+// Per-trait constants of the statistics, loaded once per thread.
+struct TraitK {
+  int ymiss;
+  int n_c;       // n_stat - ymiss - p
+  double vf;     // (n_c + p) / n_stat: SE scale of the unmasked var_x
+  double r2s;    // R2 = 1 - rss * r2s = 1 - (rss / (n_c - 1)) / (SST / (n_stat - 1))
+  double inv_nc; // sigma^2 = rss / n_c
+};
+
+__device__ __forceinline__ TraitK trait_k(const DevScan& d, int y) {
+  TraitK k;
+  k.ymiss = d.ymiss[y];
+  k.n_c = d.n_stat - k.ymiss - d.p;
+  k.vf = (double)(k.n_c + d.p) / (double)d.n_stat;
+  k.r2s = (double)(d.n_stat - 1) / ((double)(k.n_c - 1) * d.sst[y]);
+  k.inv_nc = 1.0 / (double)k.n_c;
+  return k;
+}
+
 template <int R, int LB>
-__device__ void stats_emit(const DevScan& d, int y, int s, const double* b, double rss) {
+__device__ void stats_emit(const DevScan& d, const TraitK& tk, int y, int s, const double* b, double rss) {
   const int l = d.l, p = d.p;
-  const int ymiss = d.ymiss[y];
-  const int n_c = d.n_stat - ymiss - p;
-  const double vf = (double)(n_c + p) / (double)d.n_stat;
-  const double R2 = 1.0 - ((rss / (double)(n_c - 1)) / (d.sst[y] / (double)(d.n_stat - 1)));
+  const int ymiss = tk.ymiss;
+  const double vf = tk.vf;
+  const double R2 = 1.0 - rss * tk.r2s;
   const double* vx = d.varx + (size_t)s * p;
   const size_t pair = (size_t)y * d.m_total + (size_t)(d.s0 + s);
   const bool dense = d.d_beta != nullptr;
@@ -484,8 +502,9 @@ __device__ void stats_emit(const DevScan& d, int y, int s, const double* b, doub
   bool has_snp = false;
   double sig = 0.0;
   if (rss > 0.0 && (R2 > d.min_r2_store || d.store_all)) {
-    sig = sqrt(rss / (double)n_c);
-    const double s2 = rss / (double)n_c * vf;
+    const double sg2 = rss * tk.inv_nc;
+    sig = sqrt(sg2);
+    const double s2 = sg2 * vf;
     const double tt = d.t_star * d.t_star;   // exact: t_star is a float
 #pragma unroll
     for (int j = 0; j < LB + R; ++j) {
@@ -579,7 +598,7 @@ __device__ double explicit_rss_thread(const DevScan& d, int y, int s, const doub
 // statistics.  b in the split layout (b[0..LB) covariates, b[LB..LB+R) SNP
 // terms).
 template <int R, int LB>
-__device__ __forceinline__ void finish_pair(const DevScan& d, int y, int s, bool ok, double* b,
+__device__ __forceinline__ void finish_pair(const DevScan& d, const TraitK& tk, int y, int s, bool ok, double* b,
                                             const double* rb, double rss) {
   const bool fallback = !ok || d.yzero[y] > 0 || !(rss >= RSS_GUARD * d.yy[y]);
   if (fallback) {
@@ -602,7 +621,7 @@ __device__ __forceinline__ void finish_pair(const DevScan& d, int y, int s, bool
     }
     rss = explicit_rss_thread<R, LB>(d, y, s, b);
   }
-  stats_emit<R, LB>(d, y, s, b, rss);
+  stats_emit<R, LB>(d, tk, y, s, b, rss);
 }
 
 // ---------------------------------------------------------------------------
@@ -789,21 +808,26 @@ __global__ __launch_bounds__(256) void k_epilogue(DevScan d) {
       rss += q;
     }
   }
-  finish_pair<R, LB>(d, y, s, ok, b, rb, rss);
+  finish_pair<R, LB>(d, trait_k(d, y), y, s, ok, b, rb, rss);
 }
 
 // ---------------------------------------------------------------------------
 // Few patterns (G << t): everything in the pair solve that depends only on
 // (SNP, pattern) is computed once per (SNP, pattern) instead of once per
-// pair.  With X = U_A^-T B_q, the Schur factor U_S, M = U_A^-1 X,
+// pair.  With X = U_A^-T B_q, the Schur factor U_S, M = U_A^-1 X and
 // a_y = U_A^-1 zt (per trait), the covariate coefficients are
-// b_l = a_y - M b_r, and the reference's misplaced-correction terms are
-// 2 b_l' K b_r - b_r' H b_r with K, H from the corrections.
+// b_l = a_y - M b_r.  The reference's misplaced-correction RSS terms,
+// 2 b_l' K b_r - b_r' H b_r, expand to 2 (KT' zt)' b_r - b_r' H2 b_r with
+// KT = U_A^-T K and H2 = H + 2 sym(M' K), so a pair needs b_l only when it
+// reports or re-evaluates it.  Rows j >= l of X, KT and M are zero, so a
+// pair reads its table row without branches.  Rows hold an even number of
+// doubles (16-byte aligned).
 template <int R, int LB>
 struct PatSnp {
   static constexpr int RR = R * (R + 1) / 2;
-  static constexpr int X = 0, M = LB * R, K = 2 * LB * R, US = 3 * LB * R, H = 3 * LB * R + RR,
-                       OK = 3 * LB * R + 2 * RR, W = 3 * LB * R + 2 * RR + 1;
+  static constexpr int KR = R > 1 ? LB * R : 0;   // no misplaced terms with one column
+  static constexpr int X = 0, KT = LB * R, US = LB * R + KR, H = US + RR, M = H + RR, OK = M + LB * R,
+                       W = (OK + 2) & ~1;
 };
 
 template <int R, int LB>
@@ -910,141 +934,173 @@ __global__ __launch_bounds__(256) void k_patsnp(DevScan d) {
       }
     }
   }
+  // K (the first SNP column carries every column's correction), P = M' K,
+  // then KT = U_A^-T K in place
+  double Kt[LB][R], Pm[R][R];
 #pragma unroll
   for (int j = 0; j < LB; ++j)
-    if (j < l)
+#pragma unroll
+    for (int c = 0; c < R; ++c) {
+      double k = 0.0;
+      if (R > 1 && j < l) {
+        if (c == 0) {
+#pragma unroll
+          for (int c2 = 1; c2 < R; ++c2) k += Clr[j][c2];
+        } else {
+          k = -Clr[j][c];
+        }
+      }
+      Kt[j][c] = k;
+    }
+#pragma unroll
+  for (int a = 0; a < R; ++a)
+#pragma unroll
+    for (int c = 0; c < R; ++c) {
+      double v = 0.0;
+#pragma unroll
+      for (int j = 0; j < LB; ++j) v += Mv[j][a] * Kt[j][c];
+      Pm[a][c] = v;
+    }
+#pragma unroll
+  for (int j = 0; j < LB; ++j)
+    if (j < l) {
+      const double inv = d.UA_invd[(size_t)j * G + g];
 #pragma unroll
       for (int c = 0; c < R; ++c) {
-        out[T::X + j * R + c] = X[j][c];
-        out[T::M + j * R + c] = Mv[j][c];
-        double k = 0.0;
-        if (R > 1) {
-          if (c == 0) {
+        double v = Kt[j][c];
 #pragma unroll
-            for (int c2 = 1; c2 < R; ++c2) k += Clr[j][c2];
-          } else {
-            k = -Clr[j][c];
-          }
-        }
-        out[T::K + j * R + c] = k;
+        for (int i = 0; i < j; ++i) v -= d.UA[(size_t)upk(i, j) * G + g] * Kt[i][c];
+        Kt[j][c] = v * inv;
       }
+    }
+#pragma unroll
+  for (int j = 0; j < LB; ++j)
+#pragma unroll
+    for (int c = 0; c < R; ++c) {
+      out[T::X + j * R + c] = X[j][c];   // zero for j >= l
+      out[T::M + j * R + c] = Mv[j][c];
+      if (R > 1) out[T::KT + j * R + c] = Kt[j][c];
+    }
 #pragma unroll
   for (int b2 = 0; b2 < R; ++b2)
 #pragma unroll
     for (int a = 0; a <= b2; ++a) {
       // diagonal stored as its reciprocal: the pair solves multiply
       out[T::US + upk(a, b2)] = (a == b2) ? 1.0 / Sc[upk(a, b2)] : Sc[upk(a, b2)];
-      out[T::H + upk(a, b2)] = (a >= l) ? (a == b2 ? 1.0 : 2.0) * Crr[upk(a, b2)] : 0.0;
+      const double h = (a >= l) ? (a == b2 ? 1.0 : 2.0) * Crr[upk(a, b2)] : 0.0;
+      out[T::H + upk(a, b2)] = h + 2.0 * (a == b2 ? Pm[a][a] : Pm[a][b2] + Pm[b2][a]);
     }
   out[T::OK] = ok ? 1.0 : 0.0;
+  if (T::W > T::OK + 1) out[T::W - 1] = 0.0;
 }
 
-// One thread per trait and EPI_SPT consecutive SNPs, reading the (SNP,
-// pattern) table.  A wave covers 64 traits, so each SNP's table row is
-// wave-uniform when the traits share a pattern.  (Several SNPs per thread
-// amortise the trait's loads but raise the register count from 32 to over
-// 130 and measured slower: 15 ms at 2 and 17.7 ms at 4 against 11.1 ms at 1
-// for 200,000 SNPs x 2,000 traits.)
-constexpr int EPI_SPT = 1;
-
+// One thread per (trait, SNP) pair, reading the (SNP, pattern) table.  A
+// wave covers 64 traits of one SNP.  Every load a pair needs in the common
+// case -- the trait's constants, its two C entries, its pattern's table row
+// -- is issued before the first use, so a pair costs about one memory round
+// trip; M and a_y are read only when b_l is needed.  Blocks are numbered so
+// that all trait blocks of one group of 4 SNPs run on one XCD and read the
+// SNPs' table rows from its L2.
 template <int R, int LB, typename CT>
-__global__ __launch_bounds__(256) void k_epilogue_pst(DevScan d) {
+__global__ __launch_bounds__(256) void k_epilogue_pst(DevScan d, int n_tb) {
   using T = PatSnp<R, LB>;
   constexpr int RR = T::RR;
   constexpr int P = LB + R;
   const int lane = threadIdx.x & 63;
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int s0 = (blockIdx.y * 4 + w) * EPI_SPT;
-  if (s0 >= d.mc) return;
-  const int y = blockIdx.x * 64 + lane;
+  const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
+  const int tbk = local % n_tb, sq = (local / n_tb) * 8 + xcd;
+  const int s = __builtin_amdgcn_readfirstlane(sq * 4 + (threadIdx.x >> 6));
+  if (s >= d.mc) return;
+  const int y = tbk * 64 + lane;
   if (y >= d.t) return;
   const int l = d.l;
   const int g = d.ypat[y];
-  const bool quirk = (R > 1) && d.pat_cnt[g] > 0;
+  const TraitK tk = trait_k(d, y);
   const double rss0 = d.rss0[y];
-  const bool bl_always = d.need_bl || quirk || d.yzero[y] > 0;
   const double guard = RSS_GUARD * d.yy[y];
-  double zt[LB], ay[LB];
+  const bool yzero = d.yzero[y] > 0;
+  double zt[LB];
 #pragma unroll
-  for (int j = 0; j < LB; ++j) {
-    zt[j] = (j < l) ? d.zt[(size_t)j * d.ldt + y] : 0.0;
-    ay[j] = (j < l) ? d.ay[(size_t)j * d.ldt + y] : 0.0;
-  }
+  for (int j = 0; j < LB; ++j) zt[j] = (j < l) ? d.zt[(size_t)j * d.ldt + y] : 0.0;
   const CT* C = reinterpret_cast<const CT*>(d.Cbuf);
-#pragma unroll 1
-  for (int q = 0; q < EPI_SPT; ++q) {
-    const int s = s0 + q;
-    if (s >= d.mc) break;
-    const double* tb = d.pst + ((size_t)s * d.G + g) * T::W;
-    double rb[R];
+  double rb[R];
 #pragma unroll
-    for (int c = 0; c < R; ++c) rb[c] = (double)C[(size_t)(s * R + c) * d.ldc + y];
-    const bool ok = tb[T::OK] != 0.0;
-    double b[P];
+  for (int c = 0; c < R; ++c) rb[c] = (double)C[(size_t)(s * R + c) * d.ldc + y];
+  const double* tb = d.pst + ((size_t)s * d.G + g) * T::W;
+  double X[LB][R], KT[R > 1 ? LB : 1][R], us[RR], h2[RR];
 #pragma unroll
-    for (int h = 0; h < P; ++h) b[h] = 0.0;
-    double rss = 0.0;
-    if (ok) {
-      double z[R];
+  for (int j = 0; j < LB; ++j)
 #pragma unroll
-      for (int c = 0; c < R; ++c) z[c] = rb[c];
+    for (int c = 0; c < R; ++c) {
+      X[j][c] = tb[T::X + j * R + c];
+      if (R > 1) KT[R > 1 ? j : 0][c] = tb[T::KT + j * R + c];
+    }
+#pragma unroll
+  for (int e = 0; e < RR; ++e) {
+    us[e] = tb[T::US + e];
+    h2[e] = tb[T::H + e];
+  }
+  const bool ok = tb[T::OK] != 0.0;
+  double b[P];
+#pragma unroll
+  for (int h = 0; h < P; ++h) b[h] = 0.0;
+  double rss = 0.0;
+  if (ok) {
+    double z[R], kz[R];
+#pragma unroll
+    for (int c = 0; c < R; ++c) {
+      double v = rb[c], k = 0.0;
+#pragma unroll
+      for (int j = 0; j < LB; ++j) {
+        v -= X[j][c] * zt[j];
+        if (R > 1) k += KT[R > 1 ? j : 0][c] * zt[j];
+      }
+      z[c] = v;
+      kz[c] = k;
+    }
+#pragma unroll
+    for (int i = 0; i < R; ++i) {   // U_S' z = num (diagonal reciprocals in the table)
+      double v = z[i];
+#pragma unroll
+      for (int k = 0; k < i; ++k) v -= us[upk(k, i)] * z[k];
+      z[i] = v * us[upk(i, i)];
+    }
+    double zz = 0.0;
+#pragma unroll
+    for (int c = 0; c < R; ++c) zz += z[c] * z[c];
+    double br[R];
+#pragma unroll
+    for (int i = R - 1; i >= 0; --i) {   // U_S b_r = z
+      double v = z[i];
+#pragma unroll
+      for (int k = i + 1; k < R; ++k) v -= us[upk(i, k)] * br[k];
+      br[i] = v * us[upk(i, i)];
+    }
+    rss = rss0 - zz;
+    if (R > 1) {   // misplaced-correction terms (zero for a pattern without missing rows)
+      double q = 0.0;
+#pragma unroll
+      for (int c = 0; c < R; ++c) q += 2.0 * kz[c] * br[c];
+#pragma unroll
+      for (int b2 = 0; b2 < R; ++b2)
+#pragma unroll
+        for (int a = 0; a <= b2; ++a) q -= h2[upk(a, b2)] * br[a] * br[b2];
+      rss += q;
+    }
+#pragma unroll
+    for (int c = 0; c < R; ++c) b[LB + c] = br[c];
+    if (d.need_bl || yzero || !(rss >= guard)) {   // reported, or the pair is re-evaluated
 #pragma unroll
       for (int j = 0; j < LB; ++j)
         if (j < l) {
+          double v = d.ay[(size_t)j * d.ldt + y];
 #pragma unroll
-          for (int c = 0; c < R; ++c) z[c] -= tb[T::X + j * R + c] * zt[j];
+          for (int c = 0; c < R; ++c) v -= tb[T::M + j * R + c] * br[c];
+          b[j] = v;
         }
-      double us[RR];
-#pragma unroll
-      for (int e = 0; e < RR; ++e) us[e] = tb[T::US + e];
-#pragma unroll
-      for (int i = 0; i < R; ++i) {   // U_S' z = num (diagonal reciprocals in the table)
-        double v = z[i];
-#pragma unroll
-        for (int k = 0; k < i; ++k) v -= us[upk(k, i)] * z[k];
-        z[i] = v * us[upk(i, i)];
-      }
-      double zz = 0.0;
-#pragma unroll
-      for (int c = 0; c < R; ++c) zz += z[c] * z[c];
-      double br[R];
-#pragma unroll
-      for (int i = R - 1; i >= 0; --i) {   // U_S b_r = z
-        double v = z[i];
-#pragma unroll
-        for (int k = i + 1; k < R; ++k) v -= us[upk(i, k)] * br[k];
-        br[i] = v * us[upk(i, i)];
-      }
-      rss = rss0 - zz;
-#pragma unroll
-      for (int c = 0; c < R; ++c) b[LB + c] = br[c];
-      if (bl_always || !(rss >= guard)) {
-        double qv = 0.0;
-#pragma unroll
-        for (int j = 0; j < LB; ++j)
-          if (j < l) {
-            double v = ay[j];
-#pragma unroll
-            for (int c = 0; c < R; ++c) v -= tb[T::M + j * R + c] * br[c];
-            b[j] = v;
-            if (quirk) {
-              double kv = 0.0;
-#pragma unroll
-              for (int c = 0; c < R; ++c) kv += tb[T::K + j * R + c] * br[c];
-              qv += 2.0 * v * kv;
-            }
-          }
-        if (quirk) {
-#pragma unroll
-          for (int b2 = 0; b2 < R; ++b2)
-#pragma unroll
-            for (int a = 0; a <= b2; ++a) qv -= tb[T::H + upk(a, b2)] * br[a] * br[b2];
-          rss += qv;
-        }
-      }
     }
-    finish_pair<R, LB>(d, y, s, ok, b, rb, rss);
   }
+  finish_pair<R, LB>(d, tk, y, s, ok, b, rb, rss);
 }
 
 // Explicit residual pass for the pairs the epilogue handed over: one wave
@@ -1086,7 +1142,7 @@ __global__ __launch_bounds__(256) void k_fallback(DevScan d) {
       acc += sy * sy;
     }
     acc = wave_sum(acc);
-    if (lane == 0) stats_emit<R, LB>(d, y, s, b, acc);
+    if (lane == 0) stats_emit<R, LB>(d, trait_k(d, y), y, s, b, acc);
   }
 }
 
@@ -1193,13 +1249,18 @@ void launch_patsnp(const DevScan& d, int lb, hipStream_t st) {
 #undef OA_PS
 }
 
-int patsnp_width(int r, int lb) { return 3 * lb * r + r * (r + 1) + 1; }
+int patsnp_width(int r, int lb) {
+  const int rr = r * (r + 1) / 2;
+  const int ok = lb * r + (r > 1 ? lb * r : 0) + 2 * rr + lb * r;
+  return (ok + 2) & ~1;   // PatSnp<R, LB>::W
+}
 
 void launch_epilogue_pst(const DevScan& d, int prec, int lb, hipStream_t st) {
-  dim3 grid((d.t + 63) / 64, (d.mc + 4 * EPI_SPT - 1) / (4 * EPI_SPT)), block(256);
-#define OA_EQ(R_, LB_)                                                                              \
-  if (prec == PREC_FP64) hipLaunchKernelGGL((k_epilogue_pst<R_, LB_, double>), grid, block, 0, st, d); \
-  else hipLaunchKernelGGL((k_epilogue_pst<R_, LB_, float>), grid, block, 0, st, d);
+  const int n_tb = (d.t + 63) / 64, n_sq = (d.mc + 3) / 4;
+  dim3 grid((unsigned)((long long)n_tb * ((n_sq + 7) / 8) * 8)), block(256);
+#define OA_EQ(R_, LB_)                                                                                    \
+  if (prec == PREC_FP64) hipLaunchKernelGGL((k_epilogue_pst<R_, LB_, double>), grid, block, 0, st, d, n_tb); \
+  else hipLaunchKernelGGL((k_epilogue_pst<R_, LB_, float>), grid, block, 0, st, d, n_tb);
   OA_DISPATCH_RLB(0, 0, OA_EQ)
 #undef OA_EQ
 }
