@@ -86,6 +86,24 @@ __device__ __forceinline__ double block_sum256(double v, double* sh /*[4]*/) {
   double s = sh[0] + sh[1] + sh[2] + sh[3];
   return s;
 }
+// Block-wide sums of N values at once (blockDim.x == 256): one pair of
+// barriers for all N, and the N shuffle chains interleave.  Each sum is
+// formed in the same order as block_sum256's.
+template <int N>
+__device__ __forceinline__ void block_sum256_n(double* v, double* sh /*[4 * N]*/) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int e = 0; e < N; ++e) v[e] += __shfl_xor(v[e], off, 64);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int e = 0; e < N; ++e) sh[w * N + e] = v[e];
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < N; ++e) v[e] = sh[e] + sh[N + e] + sh[2 * N + e] + sh[3 * N + e];
+}
 __device__ __forceinline__ long long block_sum256_ll(long long v, long long* sh) {
   v = wave_sum_ll(v);
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;

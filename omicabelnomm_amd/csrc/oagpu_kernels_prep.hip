@@ -1,0 +1,274 @@
+// Operand pass of the scan, one launch per chunk of SNPs: mean imputation of
+// missing genotypes (replace_nans_avgs, src/Utility.cpp:103, called at
+// src/Algorithm.cpp:622), interaction columns (src/AIOwrapper.cpp:808-838),
+// the contraction operand A (row s*R+c at stride ldop; entries n..n_pad-1
+// zero), Sbr = XR_s'XR_s (src/Algorithm.cpp:490) and -- when the patterns'
+// missing rows are few -- the imputed values at the union of those rows
+// (u-major, so the corrections kernel reads them coalesced).
+#include "oagpu_kernels.h"
+#include "oagpu_geno.h"
+#include <type_traits>
+
+namespace oa {
+
+// SNPs per block: about 4 output columns, so a thread's 4-row runs of every
+// column stay in registers at 5 waves per SIMD.
+template <int R>
+constexpr int prep_spw() { return R == 1 ? 4 : (R == 2 ? 2 : 1); }
+
+template <int OP>
+__device__ __forceinline__ void store_op4(void* Ahi, void* Alo, size_t o, const double* v) {
+  if (OP == PREC_FP64) {
+    d4 x = {v[0], v[1], v[2], v[3]};
+    *reinterpret_cast<d4*>(reinterpret_cast<double*>(Ahi) + o) = x;
+  } else if (OP == PREC_FP32) {
+    f4 x = {(float)v[0], (float)v[1], (float)v[2], (float)v[3]};
+    *reinterpret_cast<f4*>(reinterpret_cast<float*>(Ahi) + o) = x;
+  } else {
+    uint16_t h[4], lo[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) split_bf16((float)v[e], h[e], lo[e]);
+    uint2 hv = {(uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16)};
+    uint2 lv = {(uint32_t)lo[0] | ((uint32_t)lo[1] << 16), (uint32_t)lo[2] | ((uint32_t)lo[3] << 16)};
+    *reinterpret_cast<uint2*>(reinterpret_cast<uint16_t*>(Ahi) + o) = hv;
+    *reinterpret_cast<uint2*>(reinterpret_cast<uint16_t*>(Alo) + o) = lv;
+  }
+}
+
+typedef __attribute__((ext_vector_type(4))) _Float16 h4;
+
+// XT: element type of the compact rows of U (void: no compact rows written).
+__device__ __forceinline__ void store_xu(double* p, double v) { *p = v; }
+__device__ __forceinline__ void store_xu(float* p, double v) { *p = (float)v; }
+__device__ __forceinline__ void store_xu(_Float16* p, double v) { *p = (_Float16)(float)v; }
+__device__ __forceinline__ void store_xu4(double* p, const double* v) {
+  *reinterpret_cast<d4*>(p) = d4{v[0], v[1], v[2], v[3]};
+}
+__device__ __forceinline__ void store_xu4(float* p, const double* v) {
+  *reinterpret_cast<f4*>(p) = f4{(float)v[0], (float)v[1], (float)v[2], (float)v[3]};
+}
+__device__ __forceinline__ void store_xu4(_Float16* p, const double* v) {
+  *reinterpret_cast<h4*>(p) = h4{(_Float16)(float)v[0], (_Float16)(float)v[1], (_Float16)(float)v[2], (_Float16)(float)v[3]};
+}
+
+// Four raw genotypes of element type GT at rows k0..k0+3 of one source
+// column; rows >= n read as 0.  vec (wave-uniform): every run of the wave is
+// inside the column and aligned, so the wave takes one path and its loads
+// of several columns stay in flight together.
+template <typename GT> struct GVec;
+template <> struct GVec<float> { typedef f4 T; };
+template <> struct GVec<double> { typedef d4 T; };
+template <> struct GVec<_Float16> { typedef h4 T; };
+
+template <typename GT>
+__device__ __forceinline__ typename GVec<GT>::T load_g4(const DevScan& d, int src, int k0, bool vec) {
+  typedef typename GVec<GT>::T V;
+  const GT* p = static_cast<const GT*>(d.Gm) + (size_t)src * d.ldg + k0;
+  V v;
+  if (vec) {
+    v = *reinterpret_cast<const V*>(p);
+  } else {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) v[e] = (k0 + e < d.n) ? p[e] : (GT)0;
+  }
+  return v;
+}
+
+// Each thread handles runs of 4 consecutive rows, so loads and operand
+// stores are vectors.  A run's loads (genotypes of every column, environment
+// rows, subset flags, row-union positions) are all issued before any of them
+// is used, so a run costs one memory round trip: the genotype element type
+// is a template parameter, which keeps every load free of branches.
+template <int R, int OP, typename XT, typename GT>
+__global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo) {
+  constexpr bool CORR = !std::is_void<XT>::value;
+  constexpr int RR = R * (R + 1) / 2;
+  constexpr int PREP_SPW = prep_spw<R>();
+  constexpr int NV = PREP_SPW * R;
+  typedef typename GVec<GT>::T GV;
+  const int tid = threadIdx.x;
+  // Consecutive SNP blocks run on one XCD (blocks are dealt to the 8 XCDs in
+  // turn), so the 32-byte pieces that neighbouring blocks write into one
+  // compact-row cache line meet in the same L2.
+  const int per = (gridDim.x + 7) >> 3;
+  const int blk = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  const int sb = blk * PREP_SPW;
+  if (sb >= d.mc) return;   // block-uniform, before any barrier
+  __shared__ double shd[4 * (2 * NV + PREP_SPW * RR)];
+  int src[NV], env[NV];
+  bool live[PREP_SPW];
+#pragma unroll
+  for (int q = 0; q < PREP_SPW; ++q) {
+    live[q] = sb + q < d.mc;
+    const int gs = d.s0 + (live[q] ? sb + q : sb);
+    col_desc<R>(d, gs, src + q * R, env + q * R);
+  }
+  // One pass that assumes no missing genotype (the common case): missing
+  // values enter as 0 while the columns' sums and NaN counts accumulate.  A
+  // block whose columns do hold NaN repeats the pass with the means imputed,
+  // overwriting everything the first pass wrote.
+  double sum[NV], cnt[NV], mean[NV], acc[PREP_SPW * RR];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) { sum[v] = 0.0; cnt[v] = 0.0; mean[v] = 0.0; }
+  // KEEP: a subset is set (a separate copy of the loop, so that without a
+  // subset no flag load sits in front of the genotype loads)
+  auto run = [&](bool impute, auto KEEP) {
+#pragma unroll
+    for (int e = 0; e < PREP_SPW * RR; ++e) acc[e] = 0.0;
+    for (int k0 = 4 * tid; k0 < d.n_pad; k0 += 1024) {
+      // loads, in the order of first use (the memory counter retires them
+      // in issue order): subset flags and row-union positions, environment
+      // rows, then the genotypes column by column
+      // keep is padded to n_pad, so its 4-byte run is in bounds
+      uchar4 kv = make_uchar4(1, 1, 1, 1);
+      if constexpr (decltype(KEEP)::value) kv = *reinterpret_cast<const uchar4*>(d.keep + k0);
+      int4 ui = make_int4(-1, -1, -1, -1);
+      if constexpr (CORR) ui = *reinterpret_cast<const int4*>(d.uidx + k0);
+      // environment rows by column position (every SNP of an interaction
+      // set multiplies column c by the same row); E rows are padded to
+      // n_pad, so the run is in bounds
+      d4 ec[R];
+#pragma unroll
+      for (int c = 0; c < R; ++c)
+        ec[c] = env[c] >= 0 ? *reinterpret_cast<const d4*>(d.E + (size_t)env[c] * d.lde + k0) : d4{1.0, 1.0, 1.0, 1.0};
+      const int kw = __builtin_amdgcn_readfirstlane(k0 - 4 * (tid & 63));   // the wave's first run
+      const bool vec = d.g_vec && kw + 256 <= d.n;
+      GV g[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) g[v] = load_g4<GT>(d, src[v], k0, vec);
+      // rows in the analysis: inside the subset (rows >= n load as 0)
+      const bool in[4] = {kv.x != 0, kv.y != 0, kv.z != 0, kv.w != 0};
+      double x[NV][4];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        d4 ev = ec[v % R];
+        if (v >= R && env[v] != env[v % R])   // block-uniform: a descriptor layout of another shape
+          ev = env[v] >= 0 ? *reinterpret_cast<const d4*>(d.E + (size_t)env[v] * d.lde + k0) : d4{1.0, 1.0, 1.0, 1.0};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          double xv;
+          if constexpr (std::is_same<GT, _Float16>::value) xv = (double)(float)g[v][e];
+          else xv = (double)g[v][e];
+          xv *= ev[e];   // 1 without an environment row
+          if (!in[e]) {
+            xv = 0.0;
+          } else if (xv == xv) {
+            if (!impute) sum[v] += xv;
+          } else {
+            if (!impute) cnt[v] += 1.0;
+            xv = mean[v];
+          }
+          x[v][e] = xv;
+        }
+      }
+      if constexpr (CORR) {
+        // rows of U: their imputed values, u-major, for the corrections
+        const int uu[4] = {ui.x, ui.y, ui.z, ui.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (uu[e] >= 0) {
+            XT* dst = static_cast<XT*>(d.Xu) + (size_t)uu[e] * d.ldxu + (size_t)sb * R;
+            if (NV == 4 && live[PREP_SPW - 1]) {
+              // one vector store: sb * R and ldxu are multiples of 4
+              const double v4[4] = {x[0][e], x[NV > 1 ? 1 : 0][e], x[NV > 2 ? 2 : 0][e], x[NV > 3 ? 3 : 0][e]};
+              store_xu4(dst, v4);
+            } else {
+#pragma unroll
+              for (int v = 0; v < NV; ++v)
+                if (live[v / R]) store_xu(dst + v, x[v][e]);
+            }
+          }
+      }
+#pragma unroll
+      for (int q = 0; q < PREP_SPW; ++q) {
+        if (!live[q]) continue;
+#pragma unroll
+        for (int c = 0; c < R; ++c)
+          store_op4<OP>(Ahi, Alo, (size_t)((sb + q) * R + c) * d.ldop + k0, x[q * R + c]);
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+          for (int b = 0; b < R; ++b)
+#pragma unroll
+            for (int a = 0; a <= b; ++a) acc[q * RR + upk(a, b)] += x[q * R + a][e] * x[q * R + b][e];
+      }
+    }
+  };
+  if (d.keep) run(false, std::true_type{});
+  else run(false, std::false_type{});
+  // one block reduction for the column sums, NaN counts and Gram pieces; the
+  // Gram pieces are final unless some column holds NaN
+  constexpr int NA = PREP_SPW * RR;
+  double red[2 * NV + NA];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    red[v] = sum[v];
+    red[NV + v] = cnt[v];
+  }
+#pragma unroll
+  for (int e = 0; e < NA; ++e) red[2 * NV + e] = acc[e];
+  block_sum256_n<2 * NV + NA>(red, shd);
+  double missing = 0.0;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    mean[v] = red[v] / ((double)d.n_stat - red[NV + v]);
+    if (live[v / R]) missing += red[NV + v];
+  }
+  if (missing > 0.0) {   // block-uniform; each thread rewrites its own rows
+    if (d.keep) run(true, std::true_type{});
+    else run(true, std::false_type{});
+    block_sum256_n<NA>(acc, shd);
+  } else {
+#pragma unroll
+    for (int e = 0; e < NA; ++e) acc[e] = red[2 * NV + e];
+  }
+  if (tid == 0) {
+#pragma unroll
+    for (int e = 0; e < NA; ++e)
+      if (live[e / RR]) d.Sbr[(size_t)(sb + e / RR) * RR + (e % RR)] = acc[e];
+#pragma unroll
+    for (int q = 0; q < PREP_SPW; ++q)
+      if (live[q])
+#pragma unroll
+        for (int c = 0; c < R; ++c) d.col_mean[(size_t)(sb + q) * R + c] = mean[q * R + c];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+template <int R, typename XT, typename GT>
+static void prep_rxg(const DevScan& d, int prec, void* Ahi, void* Alo, hipStream_t st) {
+  const int nb = (d.mc + prep_spw<R>() - 1) / prep_spw<R>();
+  dim3 grid((nb + 7) / 8 * 8), block(256);   // a multiple of 8 for the XCD mapping
+  if (prec == PREC_FP64) hipLaunchKernelGGL((k_prep<R, PREC_FP64, XT, GT>), grid, block, 0, st, d, Ahi, Alo);
+  else if (prec == PREC_FP32) hipLaunchKernelGGL((k_prep<R, PREC_FP32, XT, GT>), grid, block, 0, st, d, Ahi, Alo);
+  else hipLaunchKernelGGL((k_prep<R, PREC_BF16X3, XT, GT>), grid, block, 0, st, d, Ahi, Alo);
+}
+
+template <int R, typename XT>
+static void prep_rx(const DevScan& d, int prec, void* Ahi, void* Alo, hipStream_t st) {
+  if (d.gdt == DT_F32) prep_rxg<R, XT, float>(d, prec, Ahi, Alo, st);
+  else if (d.gdt == DT_F64) prep_rxg<R, XT, double>(d, prec, Ahi, Alo, st);
+  else prep_rxg<R, XT, _Float16>(d, prec, Ahi, Alo, st);
+}
+
+// Narrow compact rows (the panel's own type) only occur with one column per
+// SNP.
+template <int R>
+static void prep_r(const DevScan& d, int prec, void* Ahi, void* Alo, bool corr, hipStream_t st) {
+  if (!corr) prep_rx<R, void>(d, prec, Ahi, Alo, st);
+  else if (R > 1 || d.xu_dt == DT_F64) prep_rx<R, double>(d, prec, Ahi, Alo, st);
+  else if (d.xu_dt == DT_F32) prep_rx<1, float>(d, prec, Ahi, Alo, st);
+  else prep_rx<1, _Float16>(d, prec, Ahi, Alo, st);
+}
+
+void launch_prep(const DevScan& d, int prec, void* Ahi, void* Alo, bool corr, hipStream_t st) {
+  switch (d.r) {
+    case 1: prep_r<1>(d, prec, Ahi, Alo, corr, st); break;
+    case 2: prep_r<2>(d, prec, Ahi, Alo, corr, st); break;
+    case 3: prep_r<3>(d, prec, Ahi, Alo, corr, st); break;
+    default: prep_r<4>(d, prec, Ahi, Alo, corr, st); break;
+  }
+}
+
+}  // namespace oa

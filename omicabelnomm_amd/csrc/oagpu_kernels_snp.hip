@@ -14,213 +14,9 @@
 // corrections add.  In exact arithmetic this is the reference's
 // full-matrix Cholesky solve (same pivots, same failure rule).
 #include "oagpu_kernels.h"
-#include <type_traits>
+#include "oagpu_geno.h"
 
 namespace oa {
-
-template <int R>
-__device__ __forceinline__ void col_desc(const DevScan& d, int gs, int* src, int* env) {
-#pragma unroll
-  for (int c = 0; c < R; ++c) {
-    const int idx = gs * R + c;
-    src[c] = d.col_src ? d.col_src[idx] : idx;
-    env[c] = d.col_env ? d.col_env[idx] : -1;
-  }
-}
-
-// Genotype value of output column (src, env) at row k, NaN if missing.
-__device__ __forceinline__ double geno_value(const DevScan& d, int src, int env, int k) {
-  double v = load_elem(d.Gm, d.gdt, (size_t)src * d.ldg + k);
-  if (env >= 0) v *= d.E[(size_t)env * d.lde + k];
-  return v;
-}
-
-// Output column (src, env) at rows k0..k0+3 (k0 a multiple of 4), NaN if
-// missing, 0 beyond n.  Interior runs are one vector load.
-__device__ __forceinline__ void geno_value4(const DevScan& d, int src, int env, int k0, double* x) {
-  const size_t base = (size_t)src * d.ldg + k0;
-  if (k0 + 4 <= d.n) {
-    load_elem4(d.Gm, d.gdt, d.g_vec != 0, base, x);
-  } else {
-#pragma unroll
-    for (int e = 0; e < 4; ++e) x[e] = (k0 + e < d.n) ? load_elem(d.Gm, d.gdt, base + e) : 0.0;
-  }
-  if (env >= 0) {   // E rows are padded to n_pad, so the run is always in bounds
-    const double* ep = d.E + (size_t)env * d.lde + k0;
-    const double2 a = *reinterpret_cast<const double2*>(ep);
-    const double2 b = *reinterpret_cast<const double2*>(ep + 2);
-    x[0] *= a.x; x[1] *= a.y; x[2] *= b.x; x[3] *= b.y;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Operand preparation, prep_spw<R>() SNPs per block: mean imputation, interaction
-// columns, the contraction operand A (row s*R+c at stride ldop, entries
-// n..n_pad-1 zero),
-// Sbr = XR_s'XR_s and -- when the patterns' missing rows are few -- the
-// imputed values at those rows (u-major, so the corrections kernel reads
-// them coalesced).  Each thread handles runs of 4 consecutive rows, so loads
-// and operand stores are vectors.
-// SNPs per block: about 4 output columns, so a thread's 4-row runs of every
-// column stay in registers at 5 waves per SIMD.
-template <int R>
-constexpr int prep_spw() { return R == 1 ? 4 : (R == 2 ? 2 : 1); }
-
-template <int OP>
-__device__ __forceinline__ void store_op4(void* Ahi, void* Alo, size_t o, const double* v) {
-  if (OP == PREC_FP64) {
-    d4 x = {v[0], v[1], v[2], v[3]};
-    *reinterpret_cast<d4*>(reinterpret_cast<double*>(Ahi) + o) = x;
-  } else if (OP == PREC_FP32) {
-    f4 x = {(float)v[0], (float)v[1], (float)v[2], (float)v[3]};
-    *reinterpret_cast<f4*>(reinterpret_cast<float*>(Ahi) + o) = x;
-  } else {
-    uint16_t h[4], lo[4];
-#pragma unroll
-    for (int e = 0; e < 4; ++e) split_bf16((float)v[e], h[e], lo[e]);
-    uint2 hv = {(uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16)};
-    uint2 lv = {(uint32_t)lo[0] | ((uint32_t)lo[1] << 16), (uint32_t)lo[2] | ((uint32_t)lo[3] << 16)};
-    *reinterpret_cast<uint2*>(reinterpret_cast<uint16_t*>(Ahi) + o) = hv;
-    *reinterpret_cast<uint2*>(reinterpret_cast<uint16_t*>(Alo) + o) = lv;
-  }
-}
-
-// XT: element type of the compact rows of U (void: no compact rows written).
-__device__ __forceinline__ void store_xu(double* p, double v) { *p = v; }
-__device__ __forceinline__ void store_xu(float* p, double v) { *p = (float)v; }
-__device__ __forceinline__ void store_xu(_Float16* p, double v) { *p = (_Float16)(float)v; }
-__device__ __forceinline__ void store_xu4(double* p, const double* v) {
-  *reinterpret_cast<d4*>(p) = d4{v[0], v[1], v[2], v[3]};
-}
-__device__ __forceinline__ void store_xu4(float* p, const double* v) {
-  *reinterpret_cast<f4*>(p) = f4{(float)v[0], (float)v[1], (float)v[2], (float)v[3]};
-}
-__device__ __forceinline__ void store_xu4(_Float16* p, const double* v) {
-  typedef __attribute__((ext_vector_type(4))) _Float16 h4;
-  *reinterpret_cast<h4*>(p) = h4{(_Float16)(float)v[0], (_Float16)(float)v[1], (_Float16)(float)v[2], (_Float16)(float)v[3]};
-}
-
-template <int R, int OP, typename XT>
-__global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo) {
-  constexpr bool CORR = !std::is_void<XT>::value;
-  constexpr int RR = R * (R + 1) / 2;
-  constexpr int PREP_SPW = prep_spw<R>();
-  constexpr int NV = PREP_SPW * R;
-  const int tid = threadIdx.x;
-  // Consecutive SNP blocks run on one XCD (blocks are dealt to the 8 XCDs in
-  // turn), so the 32-byte pieces that neighbouring blocks write into one
-  // compact-row cache line meet in the same L2.
-  const int per = (gridDim.x + 7) >> 3;
-  const int blk = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-  const int sb = blk * PREP_SPW;
-  if (sb >= d.mc) return;   // block-uniform, before any barrier
-  __shared__ double shd[4];
-  int src[NV], env[NV];
-  bool live[PREP_SPW];
-#pragma unroll
-  for (int q = 0; q < PREP_SPW; ++q) {
-    live[q] = sb + q < d.mc;
-    const int gs = d.s0 + (live[q] ? sb + q : sb);
-    col_desc<R>(d, gs, src + q * R, env + q * R);
-  }
-  // One pass that assumes no missing genotype (the common case): missing
-  // values enter as 0 while the columns' sums and NaN counts accumulate.  A
-  // block whose columns do hold NaN repeats the pass with the means imputed,
-  // overwriting everything the first pass wrote.
-  double sum[NV], cnt[NV], mean[NV], acc[PREP_SPW * RR];
-#pragma unroll
-  for (int v = 0; v < NV; ++v) { sum[v] = 0.0; cnt[v] = 0.0; mean[v] = 0.0; }
-  auto run = [&](bool impute) {
-#pragma unroll
-    for (int e = 0; e < PREP_SPW * RR; ++e) acc[e] = 0.0;
-    // two 4-row runs in flight per thread: the loop is latency bound
-#pragma unroll 2
-    for (int k0 = 4 * tid; k0 < d.n_pad; k0 += 1024) {
-      // rows in the analysis: below n and inside the subset (keep is padded
-      // to n_pad, so its 4-byte run is in bounds)
-      bool in[4];
-      {
-        const uchar4 kv = d.keep ? *reinterpret_cast<const uchar4*>(d.keep + k0) : make_uchar4(1, 1, 1, 1);
-        in[0] = kv.x && k0 < d.n;
-        in[1] = kv.y && k0 + 1 < d.n;
-        in[2] = kv.z && k0 + 2 < d.n;
-        in[3] = kv.w && k0 + 3 < d.n;
-      }
-      double x[NV][4];
-#pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        geno_value4(d, src[v], env[v], k0, x[v]);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const double xv = x[v][e];
-          if (!in[e]) {
-            x[v][e] = 0.0;
-          } else if (xv == xv) {
-            if (!impute) sum[v] += xv;
-          } else {
-            if (!impute) cnt[v] += 1.0;
-            x[v][e] = mean[v];
-          }
-        }
-      }
-      if constexpr (CORR) {
-        // rows of U: their imputed values, u-major, for the corrections
-        const int4 ui = *reinterpret_cast<const int4*>(d.uidx + k0);
-        const int uu[4] = {ui.x, ui.y, ui.z, ui.w};
-#pragma unroll
-        for (int e = 0; e < 4; ++e)
-          if (uu[e] >= 0) {
-            XT* dst = static_cast<XT*>(d.Xu) + (size_t)uu[e] * d.ldxu + (size_t)sb * R;
-            if (NV == 4 && live[PREP_SPW - 1]) {
-              // one vector store: sb * R and ldxu are multiples of 4
-              const double v4[4] = {x[0][e], x[NV > 1 ? 1 : 0][e], x[NV > 2 ? 2 : 0][e], x[NV > 3 ? 3 : 0][e]};
-              store_xu4(dst, v4);
-            } else {
-#pragma unroll
-              for (int v = 0; v < NV; ++v)
-                if (live[v / R]) store_xu(dst + v, x[v][e]);
-            }
-          }
-      }
-#pragma unroll
-      for (int q = 0; q < PREP_SPW; ++q) {
-        if (!live[q]) continue;
-#pragma unroll
-        for (int c = 0; c < R; ++c)
-          store_op4<OP>(Ahi, Alo, (size_t)((sb + q) * R + c) * d.ldop + k0, x[q * R + c]);
-#pragma unroll
-        for (int e = 0; e < 4; ++e)
-#pragma unroll
-          for (int b = 0; b < R; ++b)
-#pragma unroll
-            for (int a = 0; a <= b; ++a) acc[q * RR + upk(a, b)] += x[q * R + a][e] * x[q * R + b][e];
-      }
-    }
-  };
-  run(false);
-  double missing = 0.0;
-#pragma unroll
-  for (int v = 0; v < NV; ++v) {
-    const double sc = block_sum256(sum[v], shd);
-    const double nc = block_sum256(cnt[v], shd);
-    mean[v] = sc / ((double)d.n_stat - nc);
-    if (live[v / R]) missing += nc;
-  }
-  if (missing > 0.0) run(true);   // block-uniform; each thread rewrites its own rows
-#pragma unroll
-  for (int e = 0; e < PREP_SPW * RR; ++e) {
-    const double v = block_sum256(acc[e], shd);
-    const int q = e / RR;
-    if (tid == 0 && live[q]) d.Sbr[(size_t)(sb + q) * RR + (e % RR)] = v;
-  }
-  if (tid == 0) {
-#pragma unroll
-    for (int q = 0; q < PREP_SPW; ++q)
-      if (live[q])
-#pragma unroll
-        for (int c = 0; c < R; ++c) d.col_mean[(size_t)(sb + q) * R + c] = mean[q * R + c];
-  }
-}
 
 // ---------------------------------------------------------------------------
 // One thread per SNP: Str from the contraction output, and the unmasked
@@ -1174,34 +970,6 @@ __global__ __launch_bounds__(256) void k_subset_sbr(DevScan d, double* out) {
 
 // ---------------------------------------------------------------------------
 // launchers
-template <int R, typename XT>
-static void prep_rx(const DevScan& d, int prec, void* Ahi, void* Alo, hipStream_t st) {
-  const int nb = (d.mc + prep_spw<R>() - 1) / prep_spw<R>();
-  dim3 grid((nb + 7) / 8 * 8), block(256);   // a multiple of 8 for the XCD mapping
-  if (prec == PREC_FP64) hipLaunchKernelGGL((k_prep<R, PREC_FP64, XT>), grid, block, 0, st, d, Ahi, Alo);
-  else if (prec == PREC_FP32) hipLaunchKernelGGL((k_prep<R, PREC_FP32, XT>), grid, block, 0, st, d, Ahi, Alo);
-  else hipLaunchKernelGGL((k_prep<R, PREC_BF16X3, XT>), grid, block, 0, st, d, Ahi, Alo);
-}
-
-// Narrow compact rows (the panel's own type) only occur with one column per
-// SNP.
-template <int R>
-static void prep_r(const DevScan& d, int prec, void* Ahi, void* Alo, bool corr, hipStream_t st) {
-  if (!corr) prep_rx<R, void>(d, prec, Ahi, Alo, st);
-  else if (R > 1 || d.xu_dt == DT_F64) prep_rx<R, double>(d, prec, Ahi, Alo, st);
-  else if (d.xu_dt == DT_F32) prep_rx<1, float>(d, prec, Ahi, Alo, st);
-  else prep_rx<1, _Float16>(d, prec, Ahi, Alo, st);
-}
-
-void launch_prep(const DevScan& d, int prec, void* Ahi, void* Alo, bool corr, hipStream_t st) {
-  switch (d.r) {
-    case 1: prep_r<1>(d, prec, Ahi, Alo, corr, st); break;
-    case 2: prep_r<2>(d, prec, Ahi, Alo, corr, st); break;
-    case 3: prep_r<3>(d, prec, Ahi, Alo, corr, st); break;
-    default: prep_r<4>(d, prec, Ahi, Alo, corr, st); break;
-  }
-}
-
 // (R, LB) pairs with compiled kernels: R=1: LB 8/16/32; R=2: 8/16; R=3,4: 8.
 #define OA_DISPATCH_RLB(R_, LB_, FN)          \
   if (d.r == 1 && lb == 8) { FN(1, 8); }      \
