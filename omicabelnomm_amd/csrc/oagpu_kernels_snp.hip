@@ -158,30 +158,47 @@ __device__ __forceinline__ double widen(_Float16 v) { return (double)(float)v; }
 // positions and covariate rows are wave-uniform, and the 64 SNPs' values at
 // one row are contiguous.  Blocks are numbered so that the pattern groups of
 // one 64-SNP tile run back to back on one XCD and share its L2.
+// SPL consecutive SNPs per lane (one vector load per row); two when a SNP
+// has one column, so every covariate value read from LDS feeds twice the
+// multiply-adds.
+template <int R>
+constexpr int corr_spl() { return R == 1 ? 2 : 1; }
+
+template <typename XT, int N> struct XVec { typedef __attribute__((ext_vector_type(N))) XT T; };
+template <typename XT> struct XVec<XT, 1> { typedef XT T; };
+
 template <int R, int LB, typename XT>
 __global__ __launch_bounds__(256) void k_corr(DevScan d, int gq_count) {
+  constexpr int SPL = corr_spl<R>();
+  constexpr int W = SPL * R;   // Xu values per lane and row
+  typedef typename XVec<XT, W>::T XV;
   const int lane = threadIdx.x & 63;
   const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
   const int tile = (local / gq_count) * 8 + xcd;
   const int g = __builtin_amdgcn_readfirstlane((local % gq_count) * 4 + (threadIdx.x >> 6));
-  if (g >= d.G || tile * 64 >= d.mc) return;
-  const int s = tile * 64 + lane;
-  const int se = s < d.mc ? s : d.mc - 1;
+  if (g >= d.G || tile * 64 * SPL >= d.mc) return;
+  const int s = (tile * 64 + lane) * SPL;
+  // lanes past the end read the last group (the row holds ldxu >= mc * R
+  // values, rounded up to 8, so the vector stays inside it) and store nothing
+  const int se = s < d.mc ? s : (d.mc - 1) / SPL * SPL;
   const int l = d.l;
   constexpr int RR = R * (R + 1) / 2;
-  double lr[LB][R], rr[RR];
+  double lr[LB][W], rr[SPL][RR];
 #pragma unroll
   for (int j = 0; j < LB; ++j)
 #pragma unroll
-    for (int c = 0; c < R; ++c) lr[j][c] = 0.0;
+    for (int c = 0; c < W; ++c) lr[j][c] = 0.0;
 #pragma unroll
-  for (int e = 0; e < RR; ++e) rr[e] = 0.0;
+  for (int p = 0; p < SPL; ++p)
+#pragma unroll
+    for (int e = 0; e < RR; ++e) rr[p][e] = 0.0;
   // The pattern's row positions come in 64 at a time (one coalesced load,
   // then broadcast lane by lane), so each batch of CORR_BATCH rows issues its
   // genotype loads back to back instead of one dependent round trip per row.
   // The 64 rows' covariate rows are staged in LDS by the lanes (one row per
-  // lane) and read back as wave-uniform broadcasts.
-  constexpr int CORR_BATCH = 16;
+  // lane, zero past l and past the pattern's last row) and read back as
+  // wave-uniform broadcasts; the multiply-adds run without branches.
+  constexpr int CORR_BATCH = 8;
   constexpr int STAGE = LB > 16 ? 32 : 64;   // rows staged at once (LDS: 4 waves x STAGE x LB doubles)
   __shared__ double xls[4][STAGE][LB];
   double (*xw)[LB] = xls[threadIdx.x >> 6];
@@ -191,48 +208,57 @@ __global__ __launch_bounds__(256) void k_corr(DevScan d, int gq_count) {
     const int my_u = lane < nb ? d.pu_list[i0 + lane] : 0;
     if (lane < STAGE) {
 #pragma unroll
-      for (int j = 0; j < LB; ++j)
-        if (j < l) xw[lane][j] = d.XLu[(size_t)my_u * l + j];
+      for (int j = 0; j < LB; ++j) xw[lane][j] = (lane < nb && j < l) ? d.XLu[(size_t)my_u * l + j] : 0.0;
     }
     __builtin_amdgcn_wave_barrier();
     for (int b0 = 0; b0 < nb; b0 += CORR_BATCH) {
-      int uu[CORR_BATCH];
-      double xv[CORR_BATCH][R];
+      double xv[CORR_BATCH][W];
 #pragma unroll
       for (int q = 0; q < CORR_BATCH; ++q) {
-        uu[q] = __builtin_amdgcn_readlane(my_u, b0 + q);   // lanes >= nb hold row 0: weight 0 below
-        const XT* xr = static_cast<const XT*>(d.Xu) + (size_t)uu[q] * d.ldxu + (size_t)se * R;
+        // rows past nb: row 0 of U, weight 0 (covariates) and zeroed below
+        const int u = __builtin_amdgcn_readlane(my_u, (b0 + q) & 63);
+        const XT* xr = static_cast<const XT*>(d.Xu) + (size_t)u * d.ldxu + (size_t)se * R;
+        const bool ok = b0 + q < nb;
+        if constexpr (W == 2 || W == 4) {   // aligned: se * R and ldxu are multiples of W
+          const XV v = *reinterpret_cast<const XV*>(xr);
 #pragma unroll
-        for (int c = 0; c < R; ++c) xv[q][c] = widen(xr[c]);
+          for (int c = 0; c < W; ++c) xv[q][c] = ok ? widen((XT)v[c]) : 0.0;
+        } else {
+#pragma unroll
+          for (int c = 0; c < W; ++c) xv[q][c] = ok ? widen(xr[c]) : 0.0;
+        }
       }
 #pragma unroll
       for (int q = 0; q < CORR_BATCH; ++q) {
-        if (b0 + q < nb) {   // wave-uniform
-          const double* xl = xw[b0 + q];
+        const double* xl = xw[(b0 + q) & (STAGE - 1)];
 #pragma unroll
-          for (int j = 0; j < LB; ++j)
-            if (j < l) {
-              const double a = xl[j];
+        for (int j = 0; j < LB; ++j) {
+          const double a = xl[j];
 #pragma unroll
-              for (int c = 0; c < R; ++c) lr[j][c] += a * xv[q][c];
-            }
+          for (int c = 0; c < W; ++c) lr[j][c] += a * xv[q][c];
+        }
+#pragma unroll
+        for (int p = 0; p < SPL; ++p)
 #pragma unroll
           for (int b = 0; b < R; ++b)
 #pragma unroll
-            for (int a = 0; a <= b; ++a) rr[upk(a, b)] += xv[q][a] * xv[q][b];
-        }
+            for (int a = 0; a <= b; ++a) rr[p][upk(a, b)] += xv[q][p * R + a] * xv[q][p * R + b];
       }
     }
     __builtin_amdgcn_wave_barrier();
   }
-  if (s < d.mc) {
 #pragma unroll
-    for (int j = 0; j < LB; ++j)
-      if (j < l)
+  for (int p = 0; p < SPL; ++p) {
+    const int sp = s + p;
+    if (sp < d.mc) {
 #pragma unroll
-        for (int c = 0; c < R; ++c) d.Ccorr[((size_t)g * d.Q + j * R + c) * d.mc + s] = lr[j][c];
+      for (int j = 0; j < LB; ++j)
+        if (j < l)
 #pragma unroll
-    for (int e = 0; e < RR; ++e) d.Ccorr[((size_t)g * d.Q + l * R + e) * d.mc + s] = rr[e];
+          for (int c = 0; c < R; ++c) d.Ccorr[((size_t)g * d.Q + j * R + c) * d.mc + sp] = lr[j][p * R + c];
+#pragma unroll
+      for (int e = 0; e < RR; ++e) d.Ccorr[((size_t)g * d.Q + l * R + e) * d.mc + sp] = rr[p][e];
+    }
   }
 }
 
@@ -990,7 +1016,8 @@ void launch_snp_gram(const DevScan& d, int prec, int lb, hipStream_t st) {
 }
 
 void launch_corr(const DevScan& d, int lb, hipStream_t st) {
-  const int tiles = (d.mc + 63) / 64, gq = (d.G + 3) / 4;
+  const int w = 64 * (d.r == 1 ? corr_spl<1>() : 1);   // SNPs per tile
+  const int tiles = (d.mc + w - 1) / w, gq = (d.G + 3) / 4;
   dim3 grid((unsigned)(((tiles + 7) / 8) * 8 * gq)), block(256);
 #define OA_CR(R_, LB_)                                                                         \
   if (d.xu_dt == DT_F64) hipLaunchKernelGGL((k_corr<R_, LB_, double>), grid, block, 0, st, d, gq); \
