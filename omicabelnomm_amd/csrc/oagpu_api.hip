@@ -130,7 +130,7 @@ DevScan oa::make_dev(const oagpu_ctx* c) {
   d.UA = c->UA; d.UA_invd = c->UA_invd; d.pat_pd = c->pat_pd;
   d.nU = c->compact_corr ? c->nU : 0;
   d.uidx = c->uidx; d.XLu = c->XLu; d.pu_off = c->pu_off; d.pu_list = c->pu_list;
-  d.Xu = c->Xu; d.xu_dt = DT_F64; d.ldxu = c->ldxu; d.pst = c->pst;
+  d.Xu = c->Xu; d.xu_dt = xu_dtype(c); d.ldxu = c->ldxu; d.pst = c->pst;
   d.Gm = c->Gm; d.gdt = c->gdt; d.ldg = c->ldg; d.col_src = c->col_src; d.col_env = c->col_env;
   d.g_vec = c->Gm && ((uintptr_t)c->Gm % 16 == 0) && (c->ldg % 4 == 0);
   d.col_mean = c->col_mean; d.Sbr = c->Sbr; d.Str = c->Str; d.varx = c->varx; d.Ccorr = c->Ccorr;
@@ -645,7 +645,7 @@ static int ensure_chunk(oagpu_ctx* c, int mc) {
   const size_t abytes = arows * c->ldop * c->op_size;
   const size_t cbytes = (size_t)mc * r * out_cols(c) * c->c_size;
   c->ldxu = ((long long)mc * r + 7) / 8 * 8;
-  const size_t xubytes = c->compact_corr ? sizeof(double) * (size_t)c->nU * c->ldxu : 0;
+  const size_t xubytes = c->compact_corr ? xu_size(c) * (size_t)c->nU * c->ldxu : 0;
   int rc = ensure_pair_buffers(c, mc);
   if (rc) return rc;
   HIPCK(c, grow(&c->Abuf, &c->A_cap, abytes));
@@ -669,7 +669,7 @@ static int scan_enqueue(oagpu_ctx* c, bool want_dense) {
                          (size_t)r * out_cols(c) * c->c_size +
                          sizeof(double) * (size_t)c->Q * std::max(c->G, 1) +
                          sizeof(double) * (size_t)(r + r * (r + 1) / 2 + l * r + p) +
-                         (c->compact_corr ? sizeof(double) * (size_t)c->nU * r : 0) +
+                         (c->compact_corr ? xu_size(c) * (size_t)c->nU * r : 0) +
                          (c->use_pst ? sizeof(double) * (size_t)c->G * patsnp_width(r, c->lb) : 0);
   long long mc = c->prm.max_snps > 0 ? c->prm.max_snps : (long long)((12ull << 30) / per_snp);
   mc = std::max(1ll, std::min(mc, (long long)m));

@@ -102,7 +102,7 @@ struct oagpu_ctx {
   void* Abuf = nullptr;
   void *Ahi = nullptr, *Alo = nullptr, *Cbuf = nullptr;   // Ahi, Alo: views into Abuf
   double *col_mean = nullptr, *Sbr = nullptr, *Str = nullptr, *varx = nullptr, *Ccorr = nullptr;
-  double* Xu = nullptr;
+  void* Xu = nullptr;   // FP64, or FP32 below an FP64 contraction (xu_dtype)
   long long ldxu = 0;
   // few patterns per trait block: per-(SNP, pattern) solve pieces
   bool use_pst = false;
@@ -159,4 +159,10 @@ DevScan make_dev(const oagpu_ctx* c);
 // Chunk buffers of the pair side (Str, var_x, corrections, (SNP, pattern)
 // table, Sbr when a subset is set) for chunks of up to mc SNPs.
 int ensure_pair_buffers(oagpu_ctx* c, int mc);
+// Element type of the compact missing-row copy Xu: FP64 below an FP64
+// contraction; FP32 below the FP32 and split-BF16 contractions, whose own
+// rounding (2^-17 and coarser) is far above the FP32 rounding of an imputed
+// genotype (2^-24 relative), while it halves the corrections' gather traffic.
+inline int xu_dtype(const oagpu_ctx* c) { return c->prm.precision == OAGPU_FP64 ? DT_F64 : DT_F32; }
+inline size_t xu_size(const oagpu_ctx* c) { return xu_dtype(c) == DT_F64 ? sizeof(double) : sizeof(float); }
 }  // namespace oa

@@ -245,6 +245,22 @@ static void prep_rxg(const DevScan& d, int prec, void* Ahi, void* Alo, hipStream
   else hipLaunchKernelGGL((k_prep<R, PREC_BF16X3, XT, GT>), grid, block, 0, st, d, Ahi, Alo);
 }
 
+// Narrow compact rows only below the FP32 and split-BF16 contractions.
+template <int R, typename XT, typename GT>
+static void prep_rxgo(const DevScan& d, int prec, void* Ahi, void* Alo, hipStream_t st) {
+  const int nb = (d.mc + prep_spw<R>() - 1) / prep_spw<R>();
+  dim3 grid((nb + 7) / 8 * 8), block(256);
+  if (prec == PREC_FP32) hipLaunchKernelGGL((k_prep<R, PREC_FP32, XT, GT>), grid, block, 0, st, d, Ahi, Alo);
+  else hipLaunchKernelGGL((k_prep<R, PREC_BF16X3, XT, GT>), grid, block, 0, st, d, Ahi, Alo);
+}
+
+template <int R, typename XT>
+static void prep_rxo(const DevScan& d, int prec, void* Ahi, void* Alo, hipStream_t st) {
+  if (d.gdt == DT_F32) prep_rxgo<R, XT, float>(d, prec, Ahi, Alo, st);
+  else if (d.gdt == DT_F64) prep_rxgo<R, XT, double>(d, prec, Ahi, Alo, st);
+  else prep_rxgo<R, XT, _Float16>(d, prec, Ahi, Alo, st);
+}
+
 template <int R, typename XT>
 static void prep_rx(const DevScan& d, int prec, void* Ahi, void* Alo, hipStream_t st) {
   if (d.gdt == DT_F32) prep_rxg<R, XT, float>(d, prec, Ahi, Alo, st);
@@ -252,14 +268,20 @@ static void prep_rx(const DevScan& d, int prec, void* Ahi, void* Alo, hipStream_
   else prep_rxg<R, XT, _Float16>(d, prec, Ahi, Alo, st);
 }
 
-// Narrow compact rows (the panel's own type) only occur with one column per
-// SNP.
+// Compact rows: FP64, FP32 (the scan below the FP32 and split-BF16
+// contractions) or the panel's own narrow type (the request server, one
+// column per SNP, any contraction).  The element type is the one the
+// buffer was sized for (d.xu_dt).
 template <int R>
 static void prep_r(const DevScan& d, int prec, void* Ahi, void* Alo, bool corr, hipStream_t st) {
   if (!corr) prep_rx<R, void>(d, prec, Ahi, Alo, st);
-  else if (R > 1 || d.xu_dt == DT_F64) prep_rx<R, double>(d, prec, Ahi, Alo, st);
-  else if (d.xu_dt == DT_F32) prep_rx<1, float>(d, prec, Ahi, Alo, st);
-  else prep_rx<1, _Float16>(d, prec, Ahi, Alo, st);
+  else if (d.xu_dt == DT_F64) prep_rx<R, double>(d, prec, Ahi, Alo, st);
+  else if constexpr (R == 1) {
+    if (d.xu_dt == DT_F32) prep_rx<1, float>(d, prec, Ahi, Alo, st);
+    else prep_rx<1, _Float16>(d, prec, Ahi, Alo, st);
+  } else {
+    prep_rxo<R, float>(d, prec, Ahi, Alo, st);   // FP32 rows with r > 1: never below an FP64 contraction
+  }
 }
 
 void launch_prep(const DevScan& d, int prec, void* Ahi, void* Alo, bool corr, hipStream_t st) {

@@ -159,17 +159,18 @@ __device__ __forceinline__ double widen(_Float16 v) { return (double)(float)v; }
 // one row are contiguous.  Blocks are numbered so that the pattern groups of
 // one 64-SNP tile run back to back on one XCD and share its L2.
 // SPL consecutive SNPs per lane (one vector load per row); two when a SNP
-// has one column, so every covariate value read from LDS feeds twice the
-// multiply-adds.
-template <int R>
-constexpr int corr_spl() { return R == 1 ? 2 : 1; }
+// has one column and there are at most 16 covariates, so every covariate
+// value read from LDS feeds twice the multiply-adds without raising the
+// register count past 5 waves per SIMD.
+template <int R, int LB>
+constexpr int corr_spl() { return (R == 1 && LB <= 16) ? 2 : 1; }
 
 template <typename XT, int N> struct XVec { typedef __attribute__((ext_vector_type(N))) XT T; };
 template <typename XT> struct XVec<XT, 1> { typedef XT T; };
 
 template <int R, int LB, typename XT>
 __global__ __launch_bounds__(256) void k_corr(DevScan d, int gq_count) {
-  constexpr int SPL = corr_spl<R>();
+  constexpr int SPL = corr_spl<R, LB>();
   constexpr int W = SPL * R;   // Xu values per lane and row
   typedef typename XVec<XT, W>::T XV;
   const int lane = threadIdx.x & 63;
@@ -1016,13 +1017,16 @@ void launch_snp_gram(const DevScan& d, int prec, int lb, hipStream_t st) {
 }
 
 void launch_corr(const DevScan& d, int lb, hipStream_t st) {
-  const int w = 64 * (d.r == 1 ? corr_spl<1>() : 1);   // SNPs per tile
-  const int tiles = (d.mc + w - 1) / w, gq = (d.G + 3) / 4;
-  dim3 grid((unsigned)(((tiles + 7) / 8) * 8 * gq)), block(256);
-#define OA_CR(R_, LB_)                                                                         \
-  if (d.xu_dt == DT_F64) hipLaunchKernelGGL((k_corr<R_, LB_, double>), grid, block, 0, st, d, gq); \
-  else if (d.xu_dt == DT_F32) hipLaunchKernelGGL((k_corr<R_, LB_, float>), grid, block, 0, st, d, gq); \
-  else hipLaunchKernelGGL((k_corr<R_, LB_, _Float16>), grid, block, 0, st, d, gq);
+  const int gq = (d.G + 3) / 4;
+  dim3 block(256);
+#define OA_CR(R_, LB_)                                                                                  \
+  {                                                                                                     \
+    const int w = 64 * corr_spl<R_, LB_>(), tiles = (d.mc + w - 1) / w;   /* SNPs per tile */          \
+    dim3 grid((unsigned)(((tiles + 7) / 8) * 8 * gq));                                                  \
+    if (d.xu_dt == DT_F64) hipLaunchKernelGGL((k_corr<R_, LB_, double>), grid, block, 0, st, d, gq);    \
+    else if (d.xu_dt == DT_F32) hipLaunchKernelGGL((k_corr<R_, LB_, float>), grid, block, 0, st, d, gq); \
+    else hipLaunchKernelGGL((k_corr<R_, LB_, _Float16>), grid, block, 0, st, d, gq);                   \
+  }
   OA_DISPATCH_RLB(0, 0, OA_CR)
 #undef OA_CR
 }
