@@ -115,27 +115,37 @@ __global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo) {
   auto run = [&](bool impute, auto KEEP) {
 #pragma unroll
     for (int e = 0; e < PREP_SPW * RR; ++e) acc[e] = 0.0;
-    for (int k0 = 4 * tid; k0 < d.n_pad; k0 += 1024) {
-      // loads, in the order of first use (the memory counter retires them
-      // in issue order): subset flags and row-union positions, environment
-      // rows, then the genotypes column by column
-      // keep is padded to n_pad, so its 4-byte run is in bounds
-      uchar4 kv = make_uchar4(1, 1, 1, 1);
-      if constexpr (decltype(KEEP)::value) kv = *reinterpret_cast<const uchar4*>(d.keep + k0);
-      int4 ui = make_int4(-1, -1, -1, -1);
-      if constexpr (CORR) ui = *reinterpret_cast<const int4*>(d.uidx + k0);
-      // environment rows by column position (every SNP of an interaction
-      // set multiplies column c by the same row); E rows are padded to
-      // n_pad, so the run is in bounds
-      d4 ec[R];
+    // A run's loads (subset flags, row-union positions, environment rows by
+    // column position -- every SNP of an interaction set multiplies column c
+    // by the same row --, the genotypes of every column) are issued one run
+    // ahead, so the memory round trip of run i+1 overlaps the work of run i.
+    // keep, uidx and the E rows are padded to n_pad, so a run is in bounds.
+    uchar4 kvn = make_uchar4(1, 1, 1, 1);
+    int4 uin = make_int4(-1, -1, -1, -1);
+    d4 ecn[R];
+    GV gn[NV];
+    auto fetch = [&](int k) {
+      if constexpr (decltype(KEEP)::value) kvn = *reinterpret_cast<const uchar4*>(d.keep + k);
+      if constexpr (CORR) uin = *reinterpret_cast<const int4*>(d.uidx + k);
 #pragma unroll
       for (int c = 0; c < R; ++c)
-        ec[c] = env[c] >= 0 ? *reinterpret_cast<const d4*>(d.E + (size_t)env[c] * d.lde + k0) : d4{1.0, 1.0, 1.0, 1.0};
-      const int kw = __builtin_amdgcn_readfirstlane(k0 - 4 * (tid & 63));   // the wave's first run
+        ecn[c] = env[c] >= 0 ? *reinterpret_cast<const d4*>(d.E + (size_t)env[c] * d.lde + k) : d4{1.0, 1.0, 1.0, 1.0};
+      const int kw = __builtin_amdgcn_readfirstlane(k - 4 * (tid & 63));   // the wave's first run
       const bool vec = d.g_vec && kw + 256 <= d.n;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) gn[v] = load_g4<GT>(d, src[v], k, vec);
+    };
+    if (4 * tid < d.n_pad) fetch(4 * tid);
+    for (int k0 = 4 * tid; k0 < d.n_pad; k0 += 1024) {
+      const uchar4 kv = kvn;
+      const int4 ui = uin;
+      d4 ec[R];
+#pragma unroll
+      for (int c = 0; c < R; ++c) ec[c] = ecn[c];
       GV g[NV];
 #pragma unroll
-      for (int v = 0; v < NV; ++v) g[v] = load_g4<GT>(d, src[v], k0, vec);
+      for (int v = 0; v < NV; ++v) g[v] = gn[v];
+      if (k0 + 1024 < d.n_pad) fetch(k0 + 1024);
       // rows in the analysis: inside the subset (rows >= n load as 0)
       const bool in[4] = {kv.x != 0, kv.y != 0, kv.z != 0, kv.w != 0};
       double x[NV][4];
