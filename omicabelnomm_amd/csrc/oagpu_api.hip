@@ -693,12 +693,15 @@ static int scan_enqueue(oagpu_ctx* c, bool want_dense) {
     HIPCK(c, hipStreamWaitEvent(c->st, c->up_ev[c->gslot], 0));
     c->upload_pending = false;
   }
-  for (int i = 0; i < CNT_N; ++i) launch_reset_counter(c->counters, i, c->st);
+  launch_reset_counters(c->counters, CNT_N, c->st);
 
   const int nchunks = (int)((m + mc - 1) / mc);
-  for (auto& e : c->pev) (void)hipEventDestroy(e);
-  c->pev.assign((size_t)nchunks * 7, nullptr);
-  for (auto& e : c->pev) HIPCK(c, hipEventCreate(&e));
+  // timing events, kept across scans
+  while (c->pev.size() < (size_t)nchunks * 7) {
+    hipEvent_t e = nullptr;
+    HIPCK(c, hipEventCreate(&e));
+    c->pev.push_back(e);
+  }
   DevScan d = make_dev(c);
   d.need_bl = c->prm.disp_cov || want_dense;
   d.d_beta = c->pd_beta; d.d_se = c->pd_se; d.d_t = c->pd_t; d.d_p = c->pd_p; d.d_kept = c->pd_kept;

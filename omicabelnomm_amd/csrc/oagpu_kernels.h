@@ -36,6 +36,8 @@ void launch_epilogue_pst(const DevScan& d, int prec, int lb, hipStream_t st);
 int patsnp_width(int r, int lb);
 void launch_fallback(const DevScan& d, int lb, hipStream_t st);
 void launch_reset_counter(unsigned long long* c, int idx, hipStream_t st);
+// counters [0, n), n <= 64, in one launch
+void launch_reset_counters(unsigned long long* c, int n, hipStream_t st);
 // r = 1: out[s] = sum over the subset's rows (d.keep) of x^2
 void launch_subset_sbr(const DevScan& d, double* out, hipStream_t st);
 

@@ -970,6 +970,9 @@ __global__ __launch_bounds__(256) void k_fallback(DevScan d) {
 }
 
 __global__ void k_reset_counter(unsigned long long* c, int idx) { c[idx] = 0ull; }
+__global__ void k_reset_counters(unsigned long long* c, int n) {
+  if ((int)threadIdx.x < n) c[threadIdx.x] = 0ull;
+}
 
 // Sbr = x'x over the subset's rows for r = 1 (a request that shares the
 // chunk's operand but not its individuals); x is imputed with the chunk's
@@ -1073,6 +1076,10 @@ void launch_fallback(const DevScan& d, int lb, hipStream_t st) {
 
 void launch_subset_sbr(const DevScan& d, double* out, hipStream_t st) {
   hipLaunchKernelGGL(k_subset_sbr, dim3(d.mc), dim3(256), 0, st, d, out);
+}
+
+void launch_reset_counters(unsigned long long* c, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, c, n);
 }
 
 void launch_reset_counter(unsigned long long* c, int idx, hipStream_t st) {

@@ -240,7 +240,7 @@ int setup_request(oagpu_server* S, Request* R) {
   if ((rc = oagpu_load_traits(c, R->Y.data(), OAGPU_F64, OAGPU_HOST, S->panel->n, R->t, 0))) return rc;
   c->use_pst = (long long)c->G * 4 <= R->t;
   if ((rc = ensure_pair_buffers(c, S->mc))) return rc;
-  for (int i = 0; i < CNT_N; ++i) launch_reset_counter(c->counters, i, c->st);
+  launch_reset_counters(c->counters, CNT_N, c->st);
   HIPCK(c, hipStreamSynchronize(c->st));
   // the inputs are on the device now
   R->Y.clear();

@@ -143,3 +143,32 @@ def test_gemm_variant(precision, variant, oracle_mod, monkeypatch):
         db, dl = precision_report(g, o)
         print(variant, "max|db|/SE", db, "max|d-log10p|", dl, flush=True)
         assert db < 1e-3 and dl < 1e-3
+
+
+@pytest.mark.parametrize("g_dtype", ["float16", "float32", "float64"])
+@pytest.mark.parametrize("ld_pad", [0, 3])
+def test_genotype_types_and_strides(g_dtype, ld_pad, oracle_mod):
+    """Every genotype element type, with a leading dimension that breaks the
+    4-row vector runs (ld = n + 3) or not, n not a multiple of 4 or 256,
+    interaction columns, missing genotypes and shared missing-phenotype
+    patterns: FP64 parity with the oracle."""
+    import torch
+    data = synth.make(n=333, l=3, m=41, t=12, interaction=True, miss="groups", pattern_groups=3,
+                      pattern_rate=0.05, planted=6, beta=0.3, geno_nan=0.03, seed=51,
+                      g_dtype=getattr(torch, g_dtype))
+    print("case genotype", g_dtype, "ld_pad", ld_pad, flush=True)
+    from omicabelnomm_amd import Scanner
+    n = data.n
+    full = torch.zeros((data.G.shape[0], n + ld_pad), dtype=data.G.dtype, device="cuda")
+    full[:, :n] = data.G.to("cuda")
+    G = full[:, :n]
+    assert G.stride(0) == n + ld_pad
+    with Scanner(n, data.l, data.r, "fp64", min_p_store=0.1, min_p_disp=0.05, min_r2_store=-10,
+                 max_traits=64, max_hits=1 << 20) as sc:
+        sc.set_covariates(data.XL)
+        sc.set_interactions(data.E)
+        sc.load_traits(data.Y.cpu().numpy().T)
+        sc.load_genotypes(G, m=data.m, col_src=data.col_src, col_env=data.col_env)
+        g = sc.scan(dense=True)
+    o = run_oracle(oracle_mod, data, min_r2_store=-10)
+    compare_fp64(g, o)
