@@ -172,7 +172,7 @@ extern "C" int oagpu_create(const oagpu_params* prm, oagpu_ctx** out) {
   c->n = n; c->l = l; c->r = r; c->p = l + r;
   c->Q = l * r + r * (r + 1) / 2;
   c->n_pad = (n + 63) / 64 * 64;
-  c->lb = l <= 8 ? 8 : (l <= 16 ? 16 : 32);
+  c->lb = l <= 8 ? 8 : (l <= 16 ? 16 : (l <= 24 ? 24 : 32));   // 24 only occurs with r = 1
   c->op_size = prm->precision == OAGPU_FP64 ? 8 : (prm->precision == OAGPU_FP32 ? 4 : 2);
   c->c_size = prm->precision == OAGPU_FP64 ? 8 : 4;
   c->ldop = (long long)c->n_pad * (prm->precision == OAGPU_BF16X3 ? 2 : 1);

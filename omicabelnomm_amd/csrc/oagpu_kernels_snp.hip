@@ -1000,10 +1000,11 @@ __global__ __launch_bounds__(256) void k_subset_sbr(DevScan d, double* out) {
 
 // ---------------------------------------------------------------------------
 // launchers
-// (R, LB) pairs with compiled kernels: R=1: LB 8/16/32; R=2: 8/16; R=3,4: 8.
+// (R, LB) pairs with compiled kernels: R=1: LB 8/16/24/32; R=2: 8/16; R=3,4: 8.
 #define OA_DISPATCH_RLB(R_, LB_, FN)          \
   if (d.r == 1 && lb == 8) { FN(1, 8); }      \
   else if (d.r == 1 && lb == 16) { FN(1, 16); } \
+  else if (d.r == 1 && lb == 24) { FN(1, 24); } \
   else if (d.r == 1) { FN(1, 32); }           \
   else if (d.r == 2 && lb == 8) { FN(2, 8); } \
   else if (d.r == 2) { FN(2, 16); }           \

@@ -256,6 +256,7 @@ void launch_trait_solve(const DevScan& d, int lb, double* zt, double* ay, double
   switch (lb) {
     case 8: hipLaunchKernelGGL(k_trait_solve<8>, grid, block, 0, st, d, zt, ay, rss0); break;
     case 16: hipLaunchKernelGGL(k_trait_solve<16>, grid, block, 0, st, d, zt, ay, rss0); break;
+    case 24: hipLaunchKernelGGL(k_trait_solve<24>, grid, block, 0, st, d, zt, ay, rss0); break;
     default: hipLaunchKernelGGL(k_trait_solve<32>, grid, block, 0, st, d, zt, ay, rss0); break;
   }
 }

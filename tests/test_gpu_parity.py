@@ -31,6 +31,10 @@ CASES = {
     "geno_nan": (dict(n=280, l=3, m=25, t=20, miss="mcar", miss_rate=0.05, geno_nan=0.03, planted=5), dict(min_r2_store=-10)),
     "l9_r1": (dict(n=500, l=9, m=40, t=40, miss="mcar", miss_rate=0.05, planted=5), dict(disp_cov=True, min_r2_store=-10)),
     "l17_r1": (dict(n=600, l=17, m=20, t=30, miss="groups", pattern_groups=3), dict(store_all=True)),
+    "l21_r1_mcar": (dict(n=600, l=21, m=21, t=25, miss="mcar", miss_rate=0.06, planted=4),
+                    dict(disp_cov=True, min_r2_store=-10)),
+    "l27_r1": (dict(n=700, l=27, m=19, t=24, miss="groups", pattern_groups=2, planted=4),
+               dict(store_all=True, disp_cov=True)),
     # missing genotypes with shared phenotype patterns: the operand pass
     # repeats with imputed means and rewrites the compact missing-row values
     "groups_gxe_geno_nan": (dict(n=450, l=4, m=37, t=48, interaction=True, miss="groups", pattern_groups=4,
