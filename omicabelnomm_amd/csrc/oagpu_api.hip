@@ -645,7 +645,8 @@ static int ensure_chunk(oagpu_ctx* c, int mc) {
   const size_t abytes = arows * c->ldop * c->op_size;
   const size_t cbytes = (size_t)mc * r * out_cols(c) * c->c_size;
   c->ldxu = ((long long)mc * r + 7) / 8 * 8;
-  const size_t xubytes = c->compact_corr ? xu_size(c) * (size_t)c->nU * c->ldxu : 0;
+  // rows 0..nU-1 of U, then a zero row (nU) for the corrections' padding lanes
+  const size_t xubytes = c->compact_corr ? xu_size(c) * ((size_t)c->nU + 1) * c->ldxu : 0;
   int rc = ensure_pair_buffers(c, mc);
   if (rc) return rc;
   HIPCK(c, grow(&c->Abuf, &c->A_cap, abytes));
@@ -676,6 +677,9 @@ static int scan_enqueue(oagpu_ctx* c, bool want_dense) {
   mc = std::min(mc, (long long)(0x7fffffff / std::max(1, r * out_cols(c))));
   int rc = ensure_chunk(c, (int)mc);
   if (rc) return rc;
+  if (c->compact_corr)
+    HIPCK(c, hipMemsetAsync(static_cast<char*>(c->Xu) + xu_size(c) * (size_t)c->nU * c->ldxu, 0,
+                            xu_size(c) * (size_t)c->ldxu, c->st));
 
   const size_t npair = (size_t)t * m;
   if (want_dense) {

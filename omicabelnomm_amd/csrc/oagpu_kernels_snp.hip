@@ -206,7 +206,8 @@ __global__ __launch_bounds__(256) void k_corr(DevScan d, int gq_count) {
   const int beg = d.pu_off[g], end = d.pu_off[g + 1];
   for (int i0 = beg; i0 < end; i0 += STAGE) {
     const int nb = __builtin_amdgcn_readfirstlane(min(STAGE, end - i0));
-    const int my_u = lane < nb ? d.pu_list[i0 + lane] : 0;
+    // lanes past the pattern's end point at row nU of Xu, which is zero
+    const int my_u = lane < nb ? d.pu_list[i0 + lane] : d.nU;
     if (lane < STAGE) {
 #pragma unroll
       for (int j = 0; j < LB; ++j) xw[lane][j] = (lane < nb && j < l) ? d.XLu[(size_t)my_u * l + j] : 0.0;
@@ -216,17 +217,15 @@ __global__ __launch_bounds__(256) void k_corr(DevScan d, int gq_count) {
       double xv[CORR_BATCH][W];
 #pragma unroll
       for (int q = 0; q < CORR_BATCH; ++q) {
-        // rows past nb: row 0 of U, weight 0 (covariates) and zeroed below
-        const int u = __builtin_amdgcn_readlane(my_u, (b0 + q) & 63);
+        const int u = __builtin_amdgcn_readlane(my_u, (b0 + q) & 63);   // past nb: the zero row
         const XT* xr = static_cast<const XT*>(d.Xu) + (size_t)u * d.ldxu + (size_t)se * R;
-        const bool ok = b0 + q < nb;
         if constexpr (W == 2 || W == 4) {   // aligned: se * R and ldxu are multiples of W
           const XV v = *reinterpret_cast<const XV*>(xr);
 #pragma unroll
-          for (int c = 0; c < W; ++c) xv[q][c] = ok ? widen((XT)v[c]) : 0.0;
+          for (int c = 0; c < W; ++c) xv[q][c] = widen((XT)v[c]);
         } else {
 #pragma unroll
-          for (int c = 0; c < W; ++c) xv[q][c] = ok ? widen(xr[c]) : 0.0;
+          for (int c = 0; c < W; ++c) xv[q][c] = widen(xr[c]);
         }
       }
 #pragma unroll
@@ -1031,7 +1030,22 @@ void launch_corr(const DevScan& d, int lb, hipStream_t st) {
     else if (d.xu_dt == DT_F32) hipLaunchKernelGGL((k_corr<R_, LB_, float>), grid, block, 0, st, d, gq); \
     else hipLaunchKernelGGL((k_corr<R_, LB_, _Float16>), grid, block, 0, st, d, gq);                   \
   }
-  OA_DISPATCH_RLB(0, 0, OA_CR)
+  // one-column SNPs with up to 8 covariates: compiled for the exact l, so no
+  // multiply-adds run on zero padding
+  if (d.r == 1 && lb == 8) {
+    switch (d.l) {
+      case 1: OA_CR(1, 1) break;
+      case 2: OA_CR(1, 2) break;
+      case 3: OA_CR(1, 3) break;
+      case 4: OA_CR(1, 4) break;
+      case 5: OA_CR(1, 5) break;
+      case 6: OA_CR(1, 6) break;
+      case 7: OA_CR(1, 7) break;
+      default: OA_CR(1, 8) break;
+    }
+  } else {
+    OA_DISPATCH_RLB(0, 0, OA_CR)
+  }
 #undef OA_CR
 }
 

@@ -479,7 +479,10 @@ extern "C" int oagpu_server_create(oagpu_panel* P, const oagpu_server_params* pr
   SRVCK(S, hipMalloc(&S->Sbr, sizeof(double) * S->mc));
   S->ldxu = ((long long)S->mc + 7) / 8 * 8;
   S->xu_dt = (!P->has_nan && P->dtype != OAGPU_F64) ? (P->dtype == OAGPU_F32 ? DT_F32 : DT_F16) : DT_F64;
-  SRVCK(S, hipMalloc(&S->Xu, dt_bytes(S->xu_dt == DT_F64 ? OAGPU_F64 : P->dtype) * (size_t)P->n * S->ldxu));
+  // rows of all n individuals, then a zero row (n) for the corrections' padding lanes
+  const size_t xu_el = dt_bytes(S->xu_dt == DT_F64 ? OAGPU_F64 : P->dtype);
+  SRVCK(S, hipMalloc(&S->Xu, xu_el * ((size_t)P->n + 1) * S->ldxu));
+  SRVCK(S, hipMemset(static_cast<char*>(S->Xu) + xu_el * (size_t)P->n * S->ldxu, 0, xu_el * S->ldxu));
   std::vector<int> id(P->n_pad, -1);
   for (int k = 0; k < P->n; ++k) id[k] = k;
   SRVCK(S, hipMalloc(&S->uidx, sizeof(int) * id.size()));
