@@ -271,10 +271,7 @@ __device__ __forceinline__ double pvalue_ref(double t) {
   return erfc(t * 0.70710678118654752440084436210484903928);
 }
 
-// Coefficient vectors use a split layout: b[0..LB) covariates (index h < l
-// valid), b[LB..LB+R) the SNP terms (coefficient index h = l + c), so every
-// register index is a compile-time constant.
-// Per-trait constants of the statistics, loaded once per thread.
+// Per-trait constants of the statistics, computed once per thread.
 struct TraitK {
   int ymiss;
   int n_c;       // n_stat - ymiss - p
@@ -293,6 +290,9 @@ __device__ __forceinline__ TraitK trait_k(const DevScan& d, int y) {
   return k;
 }
 
+// Coefficient vectors use a split layout: b[0..LB) covariates (index h < l
+// valid), b[LB..LB+R) the SNP terms (coefficient index h = l + c), so every
+// register index is a compile-time constant.
 template <int R, int LB>
 __device__ void stats_emit(const DevScan& d, const TraitK& tk, int y, int s, const double* b, double rss) {
   const int l = d.l, p = d.p;
