@@ -447,16 +447,23 @@ __device__ __forceinline__ void finish_pair(const DevScan& d, const TraitK& tk, 
 }
 
 // ---------------------------------------------------------------------------
-// One thread per (trait, SNP) pair: a wave covers 64 traits of one SNP, so
-// the SNP's Gram pieces are wave-uniform.
+// One thread per (trait, SNP) pair when patterns are many (typically one
+// per trait): a wave covers 64 consecutive SNPs of one trait, so the trait's
+// and its pattern's data (U_A, zt, constants) are wave-uniform and the
+// pattern's corrections of the 64 SNPs are one coalesced load per term;
+// only the 64 contraction outputs are scattered (one 4- or 8-byte value per
+// SNP row).  Blocks hold 4 consecutive traits and are numbered so that the
+// blocks of one 64-SNP tile run back to back on one XCD, whose L2 then
+// serves the tile's contraction rows to all of them.
 template <int R, int LB, typename CT>
-__global__ __launch_bounds__(256) void k_epilogue(DevScan d) {
+__global__ __launch_bounds__(256) void k_epilogue(DevScan d, int n_q) {
   const int lane = threadIdx.x & 63;
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int s = blockIdx.y * 4 + w;
+  const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
+  const int q = local % n_q, tile = (local / n_q) * 8 + xcd;
+  const int y = __builtin_amdgcn_readfirstlane(q * 4 + (threadIdx.x >> 6));
+  if (y >= d.t || tile * 64 >= d.mc) return;   // wave-uniform
+  const int s = tile * 64 + lane;
   if (s >= d.mc) return;
-  const int y = blockIdx.x * 64 + lane;
-  if (y >= d.t) return;
   constexpr int RR = R * (R + 1) / 2;
   constexpr int P = LB + R;
   const int l = d.l, G = d.G;
@@ -1050,10 +1057,11 @@ void launch_corr(const DevScan& d, int lb, hipStream_t st) {
 }
 
 void launch_epilogue(const DevScan& d, int prec, int lb, hipStream_t st) {
-  dim3 grid((d.t + 63) / 64, (d.mc + 3) / 4), block(256);
-#define OA_EP(R_, LB_)                                                                          \
-  if (prec == PREC_FP64) hipLaunchKernelGGL((k_epilogue<R_, LB_, double>), grid, block, 0, st, d); \
-  else hipLaunchKernelGGL((k_epilogue<R_, LB_, float>), grid, block, 0, st, d);
+  const int n_q = (d.t + 3) / 4, tiles = (d.mc + 63) / 64;
+  dim3 grid((unsigned)((long long)((tiles + 7) / 8) * 8 * n_q)), block(256);
+#define OA_EP(R_, LB_)                                                                                \
+  if (prec == PREC_FP64) hipLaunchKernelGGL((k_epilogue<R_, LB_, double>), grid, block, 0, st, d, n_q); \
+  else hipLaunchKernelGGL((k_epilogue<R_, LB_, float>), grid, block, 0, st, d, n_q);
   OA_DISPATCH_RLB(0, 0, OA_EP)
 #undef OA_EP
 }
