@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--p-store", type=float, default=1e-6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--host-fed", type=int, default=0, metavar="BLOCKS",
+                    help="also time the scan with genotypes in page-locked host memory, uploaded in BLOCKS "
+                         "blocks, each upload overlapping the previous block's scan (configs 0-2)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 1 GiB device copy first (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     return ap.parse_args()
@@ -91,6 +94,61 @@ def timed(sc, data, steps, warmup):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
     return dt, res, gemm_ms / max(chunks, 1), chunks / steps
+
+
+def host_fed(data, precision, p_store, blocks, steps):
+    """The scan with the genotypes in page-locked host memory, as the drop-in
+    driver streams DatABEL blocks: each block is uploaded while the previous
+    block's scan runs (oagpu_scan_async), so the rate is bounded by PCIe or
+    by the scan, whichever is slower."""
+    import torch
+    m, r = data.m, data.r
+    mb = (m + blocks - 1) // blocks
+    parts = []
+    for s0 in range(0, m, mb):
+        s1 = min(m, s0 + mb)
+        if data.col_src is None:
+            lo, hi, src, env = s0 * r, s1 * r, None, None
+        else:
+            src = np.asarray(data.col_src[s0 * r:s1 * r])
+            lo, hi = int(src.min()), int(src.max()) + 1
+            assert hi - lo <= (s1 - s0) * r
+            src = src - lo
+            env = None if data.col_env is None else np.asarray(data.col_env[s0 * r:s1 * r])
+        hb = torch.empty((hi - lo, data.n), dtype=data.G.dtype, pin_memory=True)
+        hb.copy_(data.G[lo:hi])
+        parts.append((hb, s1 - s0, src, env, s0))
+    # PCIe rate of one block, for the record
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dev = parts[0][0].to("cuda", non_blocking=True)
+    torch.cuda.synchronize()
+    h2d = parts[0][0].numel() * parts[0][0].element_size() / (time.perf_counter() - t0) / 1e9
+    del dev
+    sc = make_scanner(data, precision, p_store)
+
+    def step():
+        sc.load_traits(data.Y)
+        hb, mcnt, src, env, s0 = parts[0]
+        sc.load_genotypes(hb.numpy().T, m=mcnt, col_src=src, col_env=env, snp_offset=s0)
+        sc.scan_async()
+        for hb, mcnt, src, env, s0 in parts[1:]:
+            sc.load_genotypes(hb.numpy().T, m=mcnt, col_src=src, col_env=env, snp_offset=s0)
+            sc.scan_wait()
+            sc.scan_async()
+        sc.scan_wait()
+
+    step()   # warm-up (autotuner, buffers)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    sc.close()
+    return {"value": m * data.t / dt, "ms_per_step": dt * 1e3, "blocks": len(parts),
+            "block_gb": round(parts[0][0].numel() * parts[0][0].element_size() / 1e9, 3),
+            "h2d_gb_per_s": round(h2d, 1)}
 
 
 def measured_traffic(variant, config):
@@ -330,6 +388,8 @@ def main():
                 "hits_p_lt_1e-8_identical": a == b, "n_hits_p_lt_1e-8": len(b),
                 "max_abs_dlog10p": dl, "max_dbeta_over_se": db}
 
+    hostfed = host_fed(data, args.precision, args.p_store, args.host_fed, args.steps) if args.host_fed else None
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(data, args.cpu_seconds)
@@ -360,6 +420,8 @@ def main():
         "timings_ms": timings,
         "stats": {k: int(v) if isinstance(v, (int, np.integer)) else v for k, v in res.stats.items()},
     }
+    if hostfed:
+        line["host_fed"] = hostfed
     print(json.dumps(line))
 
 

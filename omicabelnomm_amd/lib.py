@@ -114,6 +114,8 @@ def load():
     L.oagpu_load_traits.argtypes = [vp, vp, i, i, ll, i, i]
     L.oagpu_load_genotypes.argtypes = [vp, vp, i, i, ll, i, i, vp, vp, i]
     L.oagpu_scan.argtypes = [vp, ctypes.POINTER(Dense), vp, ll, ctypes.POINTER(Stats)]
+    L.oagpu_scan_async.argtypes = [vp, i]
+    L.oagpu_scan_wait.argtypes = [vp, ctypes.POINTER(Dense), vp, ll, ctypes.POINTER(Stats)]
     L.oagpu_last_timings.argtypes = [vp, ctypes.POINTER(ctypes.c_double), i]
     L.oagpu_gemm_variant.argtypes = [vp]
     L.oagpu_gemm_variant.restype = ctypes.c_char_p

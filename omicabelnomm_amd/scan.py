@@ -325,6 +325,25 @@ class Scanner:
         return ScanResult(hits=hits[: stats["n_hits"]].copy(), stats=stats, dense=dres,
                           timings_ms=self.timings())
 
+    def scan_async(self) -> None:
+        """Enqueue the scan of the loaded traits and genotypes and return; the
+        next genotype block may be loaded (its upload overlaps this scan)
+        before scan_wait collects the hits (oagpu_scan_async)."""
+        self._check(self._L.oagpu_scan_async(self._ctx, 0))
+
+    def scan_wait(self, hit_cap=None) -> ScanResult:
+        """Hits and statistics of the scan started by scan_async."""
+        cap = int(hit_cap if hit_cap is not None else self.params.max_hits)
+        hits = np.zeros(cap, dtype=L.HIT_DTYPE)
+        st = L.Stats()
+        rc = self._L.oagpu_scan_wait(self._ctx, None, hits.ctypes.data, cap, ctypes.byref(st))
+        stats = {k: getattr(st, k) for k, _ in L.Stats._fields_}
+        if rc == L.ERR_HIT_OVERFLOW:
+            raise L.OagpuError(rc, self._L.oagpu_last_error(self._ctx).decode() + " stats=%s" % stats)
+        self._check(rc)
+        return ScanResult(hits=hits[: stats["n_hits"]].copy(), stats=stats, dense=None,
+                          timings_ms=self.timings())
+
     def gemm_variant(self) -> str:
         return self._L.oagpu_gemm_variant(self._ctx).decode()
 
