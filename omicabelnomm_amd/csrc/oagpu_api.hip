@@ -634,7 +634,7 @@ int oa::ensure_pair_buffers(oagpu_ctx* c, int mc) {
   HIPCK(c, grow((void**)&c->Ccorr, &c->Ccorr_cap, qbytes));
   if (c->use_pst)
     HIPCK(c, grow((void**)&c->pst, &c->pst_cap,
-                  sizeof(double) * (size_t)mc * G * patsnp_width(r, c->lb)));
+                  sizeof(double) * (size_t)mc * G * patsnp_width(r, c->l, c->lb)));
   return OAGPU_OK;
 }
 
@@ -671,7 +671,7 @@ static int scan_enqueue(oagpu_ctx* c, bool want_dense) {
                          sizeof(double) * (size_t)c->Q * std::max(c->G, 1) +
                          sizeof(double) * (size_t)(r + r * (r + 1) / 2 + l * r + p) +
                          (c->compact_corr ? xu_size(c) * (size_t)c->nU * r : 0) +
-                         (c->use_pst ? sizeof(double) * (size_t)c->G * patsnp_width(r, c->lb) : 0);
+                         (c->use_pst ? sizeof(double) * (size_t)c->G * patsnp_width(r, c->l, c->lb) : 0);
   long long mc = c->prm.max_snps > 0 ? c->prm.max_snps : (long long)((12ull << 30) / per_snp);
   mc = std::max(1ll, std::min(mc, (long long)m));
   mc = std::min(mc, (long long)(0x7fffffff / std::max(1, r * out_cols(c))));

@@ -29,11 +29,11 @@ void launch_snp_gram(const DevScan& d, int prec, int lb, hipStream_t st);
 void launch_corr(const DevScan& d, int lb, hipStream_t st);
 // one thread per pair, reading the corrections from d.Ccorr
 void launch_epilogue(const DevScan& d, int prec, int lb, hipStream_t st);
-// Few patterns: per-(SNP, pattern) solve pieces (d.pst, patsnp_width
-// doubles each), then the per-pair epilogue that reads them.
+// Few patterns: per-(SNP, pattern) solve pieces (d.pst, patsnp_width(r, l,
+// lb) doubles each), then the per-pair epilogue that reads them.
 void launch_patsnp(const DevScan& d, int lb, hipStream_t st);
 void launch_epilogue_pst(const DevScan& d, int prec, int lb, hipStream_t st);
-int patsnp_width(int r, int lb);
+int patsnp_width(int r, int l, int lb);
 void launch_fallback(const DevScan& d, int lb, hipStream_t st);
 void launch_reset_counter(unsigned long long* c, int idx, hipStream_t st);
 // counters [0, n), n <= 64, in one launch

@@ -15,6 +15,7 @@
 // full-matrix Cholesky solve (same pivots, same failure rule).
 #include "oagpu_kernels.h"
 #include "oagpu_geno.h"
+#include "oagpu_pair.h"
 
 namespace oa {
 
@@ -263,190 +264,6 @@ __global__ __launch_bounds__(256) void k_corr(DevScan d, int gq_count) {
 }
 
 // ---------------------------------------------------------------------------
-// hpc_statistics for one (trait, SNP) pair (src/Algorithm.cpp:1034-1214):
-// R2, SE from the unmasked var_x scaled by (n - Ymiss)/n, t, the
-// normal-approximation P, thresholds, counts, and the hit records.
-__device__ __forceinline__ double pvalue_ref(double t) {
-  if (t < 1.6) return 0.5 - 0.1 * (4.4 - t) * t;
-  return erfc(t * 0.70710678118654752440084436210484903928);
-}
-
-// Per-trait constants of the statistics, computed once per thread.
-struct TraitK {
-  int ymiss;
-  int n_c;       // n_stat - ymiss - p
-  double vf;     // (n_c + p) / n_stat: SE scale of the unmasked var_x
-  double r2s;    // R2 = 1 - rss * r2s = 1 - (rss / (n_c - 1)) / (SST / (n_stat - 1))
-  double inv_nc; // sigma^2 = rss / n_c
-};
-
-__device__ __forceinline__ TraitK trait_k(const DevScan& d, int y) {
-  TraitK k;
-  k.ymiss = d.ymiss[y];
-  k.n_c = d.n_stat - k.ymiss - d.p;
-  k.vf = (double)(k.n_c + d.p) / (double)d.n_stat;
-  k.r2s = (double)(d.n_stat - 1) / ((double)(k.n_c - 1) * d.sst[y]);
-  k.inv_nc = 1.0 / (double)k.n_c;
-  return k;
-}
-
-// Coefficient vectors use a split layout: b[0..LB) covariates (index h < l
-// valid), b[LB..LB+R) the SNP terms (coefficient index h = l + c), so every
-// register index is a compile-time constant.
-template <int R, int LB>
-__device__ void stats_emit(const DevScan& d, const TraitK& tk, int y, int s, const double* b, double rss) {
-  const int l = d.l, p = d.p;
-  const int ymiss = tk.ymiss;
-  const double vf = tk.vf;
-  const double R2 = 1.0 - rss * tk.r2s;
-  const double* vx = d.varx + (size_t)s * p;
-  const size_t pair = (size_t)y * d.m_total + (size_t)(d.s0 + s);
-  const bool dense = d.d_beta != nullptr;
-  const double qnan = __longlong_as_double(0x7ff8000000000000ll);
-  if (dense) {
-    d.d_r2[pair] = R2;
-    d.d_rss[pair] = rss;
-#pragma unroll
-    for (int j = 0; j < LB + R; ++j) {
-      const int h = j < LB ? j : l + (j - LB);
-      if (j >= LB || j < l) {
-        d.d_beta[pair * p + h] = b[j];
-        d.d_se[pair * p + h] = qnan;
-        d.d_t[pair * p + h] = qnan;
-        d.d_p[pair * p + h] = qnan;
-        d.d_kept[pair * p + h] = 0;
-      }
-    }
-  }
-  // kept[j] for slot j of the split layout
-  unsigned long long mask = 0;
-  int nk = 0, nsig = 0;
-  bool has_snp = false;
-  double sig = 0.0;
-  if (rss > 0.0 && (R2 > d.min_r2_store || d.store_all)) {
-    const double sg2 = rss * tk.inv_nc;
-    sig = sqrt(sg2);
-    const double s2 = sg2 * vf;
-    const double tt = d.t_star * d.t_star;   // exact: t_star is a float
-#pragma unroll
-    for (int j = 0; j < LB + R; ++j) {
-      const int h = j < LB ? j : l + (j - LB);
-      if (j < LB && (j >= l || j < d.start_h)) continue;
-      const double vxh = vx[h];
-      if (vxh < 0.0) atomicOr(&d.counters[CNT_ERR], 1ull);
-      // |b/se| > t_star  <=>  b^2 > t_star^2 se^2 (se^2 >= 0): the square
-      // roots and the division are only needed for what is kept
-      const double se2 = s2 * vxh;
-      if (!(dense || d.store_all || (se2 >= 0.0 && b[j] * b[j] > tt * se2))) continue;
-      const double se = sig * sqrt(vxh * vf);
-      const double tv = fabs(b[j] / se);
-      if (dense) {
-        d.d_se[pair * p + h] = se;
-        d.d_t[pair * p + h] = tv;
-      }
-      if (tv > d.t_star || d.store_all) {
-        mask |= 1ull << j;
-        ++nk;
-        if (j >= LB) has_snp = true;
-        const double pv = pvalue_ref(tv);
-        if (d.min_p_disp > pv) ++nsig;
-        if (dense) {
-          d.d_p[pair * p + h] = pv;
-          d.d_kept[pair * p + h] = 1;
-        }
-      }
-    }
-  }
-  if (nsig) atomicAdd(&d.counters[CNT_SIG], (unsigned long long)nsig);
-  if (dense) d.d_emitted[pair] = has_snp ? 1 : 0;
-  if (!has_snp) return;
-  atomicAdd(&d.counters[CNT_EMITTED], 1ull);
-  const unsigned long long base = atomicAdd(&d.counters[CNT_HITS], (unsigned long long)nk);
-  if (base + nk > (unsigned long long)d.hit_cap) return;
-  unsigned long long o = base;
-  // SNP terms then covariates, each descending: h descending overall
-#pragma unroll
-  for (int jj = 0; jj < LB + R; ++jj) {
-    const int j = jj < R ? LB + R - 1 - jj : LB - 1 - (jj - R);
-    if (!((mask >> j) & 1ull)) continue;
-    const int h = j < LB ? j : l + (j - LB);
-    const double se = sig * sqrt(vx[h] * vf);
-    const double tv = fabs(b[j] / se);
-    oagpu_hit hr;
-    hr.trait = d.trait_offset + y;
-    hr.snp = d.snp_offset + d.s0 + s;
-    hr.h = h;
-    hr.n_used = d.n_stat - ymiss;
-    hr.b = b[j];
-    hr.se = se;
-    hr.t = tv;
-    hr.p = pvalue_ref(tv);
-    hr.r2 = R2;
-    d.hits[o++] = hr;
-  }
-}
-
-// RSS over rows with y0 != 0 (src/Algorithm.cpp:976-1024), one thread.
-template <int R, int LB>
-__device__ double explicit_rss_thread(const DevScan& d, int y, int s, const double* b) {
-  int src[R], env[R];
-  col_desc<R>(d, d.s0 + s, src, env);
-  double mean[R];
-#pragma unroll
-  for (int c = 0; c < R; ++c) mean[c] = d.col_mean[(size_t)s * R + c];
-  double acc = 0.0;
-  for (int k = 0; k < d.n; ++k) {
-    if (d.keep && !d.keep[k]) continue;
-    double yv = load_elem(d.Y, d.ydt, (size_t)y * d.ldy + k);
-    if (yv != yv) yv = 0.0;
-    if (yv == 0.0) continue;
-    double sy = 0.0;
-#pragma unroll
-    for (int h = 0; h < LB; ++h)
-      if (h < d.l) sy += d.XL_cm[(size_t)h * d.n_pad + k] * b[h];
-#pragma unroll
-    for (int c = 0; c < R; ++c) {
-      double v = geno_value(d, src[c], env[c], k);
-      if (v != v) v = mean[c];
-      sy += v * b[LB + c];
-    }
-    sy -= yv;
-    acc += sy * sy;
-  }
-  return acc;
-}
-
-// Shared end of a pair: the hand-over to the explicit residual pass and the
-// statistics.  b in the split layout (b[0..LB) covariates, b[LB..LB+R) SNP
-// terms).
-template <int R, int LB>
-__device__ __forceinline__ void finish_pair(const DevScan& d, const TraitK& tk, int y, int s, bool ok, double* b,
-                                            const double* rb, double rss) {
-  const bool fallback = !ok || d.yzero[y] > 0 || !(rss >= RSS_GUARD * d.yy[y]);
-  if (fallback) {
-    if (!ok) {   // sposv left the right-hand side in place
-#pragma unroll
-      for (int h = 0; h < LB; ++h) b[h] = (h < d.l) ? d.rt[(size_t)h * d.ldt + y] : 0.0;
-#pragma unroll
-      for (int c = 0; c < R; ++c) b[LB + c] = rb[c];
-    }
-    const unsigned long long slot = atomicAdd(&d.counters[CNT_FB_CHUNK], 1ull);
-    atomicAdd(&d.counters[CNT_FALLBACK], 1ull);
-    if (slot < (unsigned long long)d.fb_cap) {
-      FallbackEntry* f = d.fb + slot;
-      f->y = y;
-      f->s = s;
-      f->flags = ok ? 0 : 1;
-#pragma unroll
-      for (int h = 0; h < LB + R; ++h) f->b[h] = b[h];
-      return;
-    }
-    rss = explicit_rss_thread<R, LB>(d, y, s, b);
-  }
-  stats_emit<R, LB>(d, tk, y, s, b, rss);
-}
-
-// ---------------------------------------------------------------------------
 // One thread per (trait, SNP) pair when patterns are many (typically one
 // per trait): a wave covers 64 consecutive SNPs of one trait, so the trait's
 // and its pattern's data (U_A, zt, constants) are wave-uniform and the
@@ -640,298 +457,6 @@ __global__ __launch_bounds__(256) void k_epilogue(DevScan d, int n_q) {
   finish_pair<R, LB>(d, trait_k(d, y), y, s, ok, b, rb, rss);
 }
 
-// ---------------------------------------------------------------------------
-// Few patterns (G << t): everything in the pair solve that depends only on
-// (SNP, pattern) is computed once per (SNP, pattern) instead of once per
-// pair.  With X = U_A^-T B_q, the Schur factor U_S, M = U_A^-1 X and
-// a_y = U_A^-1 zt (per trait), the covariate coefficients are
-// b_l = a_y - M b_r.  The reference's misplaced-correction RSS terms,
-// 2 b_l' K b_r - b_r' H b_r, expand to 2 (KT' zt)' b_r - b_r' H2 b_r with
-// KT = U_A^-T K and H2 = H + 2 sym(M' K), so a pair needs b_l only when it
-// reports or re-evaluates it.  Rows j >= l of X, KT and M are zero, so a
-// pair reads its table row without branches.  Rows hold an even number of
-// doubles (16-byte aligned).
-template <int R, int LB>
-struct PatSnp {
-  static constexpr int RR = R * (R + 1) / 2;
-  static constexpr int KR = R > 1 ? LB * R : 0;   // no misplaced terms with one column
-  static constexpr int X = 0, KT = LB * R, US = LB * R + KR, H = US + RR, M = H + RR, OK = M + LB * R,
-                       W = (OK + 2) & ~1;
-};
-
-template <int R, int LB>
-__global__ __launch_bounds__(256) void k_patsnp(DevScan d) {
-  using T = PatSnp<R, LB>;
-  constexpr int RR = T::RR;
-  const int G = d.G, l = d.l;
-  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (idx >= (long long)d.mc * G) return;
-  // SNP fastest: the corrections of one pattern are contiguous in SNPs
-  const int g = (int)(idx / d.mc), s = (int)(idx % d.mc);
-  const int cnt = d.pat_cnt[g];
-  double X[LB][R], Clr[LB][R], Crr[RR];
-#pragma unroll
-  for (int j = 0; j < LB; ++j)
-#pragma unroll
-    for (int c = 0; c < R; ++c) {
-      X[j][c] = (j < l) ? d.Str[((size_t)s * l + j) * R + c] : 0.0;
-      Clr[j][c] = 0.0;
-    }
-#pragma unroll
-  for (int e = 0; e < RR; ++e) Crr[e] = 0.0;
-  if (cnt > 0) {
-    // every column's correction is subtracted from the first SNP column
-    // (src/Algorithm.cpp:1390-1396)
-#pragma unroll
-    for (int j = 0; j < LB; ++j)
-      if (j < l) {
-        double tot = 0.0;
-#pragma unroll
-        for (int c = 0; c < R; ++c) {
-          const double v = d.Ccorr[((size_t)g * d.Q + j * R + c) * d.mc + s];
-          tot += v;
-          Clr[j][c] = v;
-        }
-        X[j][0] -= tot;
-      }
-#pragma unroll
-    for (int e = 0; e < RR; ++e) Crr[e] = d.Ccorr[((size_t)g * d.Q + l * R + e) * d.mc + s];
-  }
-  // D_q: Sbr corrected only in rows a < min(l, r) (src/Algorithm.cpp:1300)
-  double Sc[RR];
-#pragma unroll
-  for (int b2 = 0; b2 < R; ++b2)
-#pragma unroll
-    for (int a = 0; a <= b2; ++a)
-      Sc[upk(a, b2)] = d.Sbr[(size_t)s * RR + upk(a, b2)] - ((a < l) ? Crr[upk(a, b2)] : 0.0);
-  // X = U_A^-T B_q
-#pragma unroll
-  for (int j = 0; j < LB; ++j)
-    if (j < l) {
-      const double inv = d.UA_invd[(size_t)j * G + g];
-#pragma unroll
-      for (int c = 0; c < R; ++c) {
-        double v = X[j][c];
-#pragma unroll
-        for (int i = 0; i < j; ++i) v -= d.UA[(size_t)upk(i, j) * G + g] * X[i][c];
-        X[j][c] = v * inv;
-      }
-    }
-  // Schur complement and its Cholesky factor
-#pragma unroll
-  for (int b2 = 0; b2 < R; ++b2)
-#pragma unroll
-    for (int a = 0; a <= b2; ++a) {
-      double v = Sc[upk(a, b2)];
-#pragma unroll
-      for (int j = 0; j < LB; ++j) v -= X[j][a] * X[j][b2];
-      Sc[upk(a, b2)] = v;
-    }
-  bool ok = d.pat_pd[g] != 0;
-#pragma unroll
-  for (int j = 0; j < R; ++j) {
-    double ajj = Sc[upk(j, j)];
-#pragma unroll
-    for (int k = 0; k < j; ++k) ajj -= Sc[upk(k, j)] * Sc[upk(k, j)];
-    if (!(ajj > 0.0)) ok = false;
-    ajj = sqrt(ajj);
-    Sc[upk(j, j)] = ajj;
-#pragma unroll
-    for (int i = j + 1; i < R; ++i) {
-      double aji = Sc[upk(j, i)];
-#pragma unroll
-      for (int k = 0; k < j; ++k) aji -= Sc[upk(k, j)] * Sc[upk(k, i)];
-      Sc[upk(j, i)] = aji / ajj;
-    }
-  }
-  double* out = d.pst + ((size_t)s * G + g) * T::W;
-  // M = U_A^-1 X
-  double Mv[LB][R];
-#pragma unroll
-  for (int j = LB - 1; j >= 0; --j) {
-#pragma unroll
-    for (int c = 0; c < R; ++c) Mv[j][c] = 0.0;
-    if (j < l) {
-      const double inv = d.UA_invd[(size_t)j * G + g];
-#pragma unroll
-      for (int c = 0; c < R; ++c) {
-        double v = X[j][c];
-#pragma unroll
-        for (int i = j + 1; i < LB; ++i)
-          if (i < l) v -= d.UA[(size_t)upk(j, i) * G + g] * Mv[i][c];
-        Mv[j][c] = v * inv;
-      }
-    }
-  }
-  // K (the first SNP column carries every column's correction), P = M' K,
-  // then KT = U_A^-T K in place
-  double Kt[LB][R], Pm[R][R];
-#pragma unroll
-  for (int j = 0; j < LB; ++j)
-#pragma unroll
-    for (int c = 0; c < R; ++c) {
-      double k = 0.0;
-      if (R > 1 && j < l) {
-        if (c == 0) {
-#pragma unroll
-          for (int c2 = 1; c2 < R; ++c2) k += Clr[j][c2];
-        } else {
-          k = -Clr[j][c];
-        }
-      }
-      Kt[j][c] = k;
-    }
-#pragma unroll
-  for (int a = 0; a < R; ++a)
-#pragma unroll
-    for (int c = 0; c < R; ++c) {
-      double v = 0.0;
-#pragma unroll
-      for (int j = 0; j < LB; ++j) v += Mv[j][a] * Kt[j][c];
-      Pm[a][c] = v;
-    }
-#pragma unroll
-  for (int j = 0; j < LB; ++j)
-    if (j < l) {
-      const double inv = d.UA_invd[(size_t)j * G + g];
-#pragma unroll
-      for (int c = 0; c < R; ++c) {
-        double v = Kt[j][c];
-#pragma unroll
-        for (int i = 0; i < j; ++i) v -= d.UA[(size_t)upk(i, j) * G + g] * Kt[i][c];
-        Kt[j][c] = v * inv;
-      }
-    }
-#pragma unroll
-  for (int j = 0; j < LB; ++j)
-#pragma unroll
-    for (int c = 0; c < R; ++c) {
-      out[T::X + j * R + c] = X[j][c];   // zero for j >= l
-      out[T::M + j * R + c] = Mv[j][c];
-      if (R > 1) out[T::KT + j * R + c] = Kt[j][c];
-    }
-#pragma unroll
-  for (int b2 = 0; b2 < R; ++b2)
-#pragma unroll
-    for (int a = 0; a <= b2; ++a) {
-      // diagonal stored as its reciprocal: the pair solves multiply
-      out[T::US + upk(a, b2)] = (a == b2) ? 1.0 / Sc[upk(a, b2)] : Sc[upk(a, b2)];
-      const double h = (a >= l) ? (a == b2 ? 1.0 : 2.0) * Crr[upk(a, b2)] : 0.0;
-      out[T::H + upk(a, b2)] = h + 2.0 * (a == b2 ? Pm[a][a] : Pm[a][b2] + Pm[b2][a]);
-    }
-  out[T::OK] = ok ? 1.0 : 0.0;
-  if (T::W > T::OK + 1) out[T::W - 1] = 0.0;
-}
-
-// One thread per (trait, SNP) pair, reading the (SNP, pattern) table.  A
-// wave covers 64 traits of one SNP.  Every load a pair needs in the common
-// case -- the trait's constants, its two C entries, its pattern's table row
-// -- is issued before the first use, so a pair costs about one memory round
-// trip; M and a_y are read only when b_l is needed.  Blocks are numbered so
-// that all trait blocks of one group of 4 SNPs run on one XCD and read the
-// SNPs' table rows from its L2.
-template <int R, int LB, typename CT>
-__global__ __launch_bounds__(256) void k_epilogue_pst(DevScan d, int n_tb) {
-  using T = PatSnp<R, LB>;
-  constexpr int RR = T::RR;
-  constexpr int P = LB + R;
-  const int lane = threadIdx.x & 63;
-  const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
-  const int tbk = local % n_tb, sq = (local / n_tb) * 8 + xcd;
-  const int s = __builtin_amdgcn_readfirstlane(sq * 4 + (threadIdx.x >> 6));
-  if (s >= d.mc) return;
-  const int y = tbk * 64 + lane;
-  if (y >= d.t) return;
-  const int l = d.l;
-  const int g = d.ypat[y];
-  const TraitK tk = trait_k(d, y);
-  const double rss0 = d.rss0[y];
-  const double guard = RSS_GUARD * d.yy[y];
-  const bool yzero = d.yzero[y] > 0;
-  double zt[LB];
-#pragma unroll
-  for (int j = 0; j < LB; ++j) zt[j] = (j < l) ? d.zt[(size_t)j * d.ldt + y] : 0.0;
-  const CT* C = reinterpret_cast<const CT*>(d.Cbuf);
-  double rb[R];
-#pragma unroll
-  for (int c = 0; c < R; ++c) rb[c] = (double)C[(size_t)(s * R + c) * d.ldc + y];
-  const double* tb = d.pst + ((size_t)s * d.G + g) * T::W;
-  double X[LB][R], KT[R > 1 ? LB : 1][R], us[RR], h2[RR];
-#pragma unroll
-  for (int j = 0; j < LB; ++j)
-#pragma unroll
-    for (int c = 0; c < R; ++c) {
-      X[j][c] = tb[T::X + j * R + c];
-      if (R > 1) KT[R > 1 ? j : 0][c] = tb[T::KT + j * R + c];
-    }
-#pragma unroll
-  for (int e = 0; e < RR; ++e) {
-    us[e] = tb[T::US + e];
-    h2[e] = tb[T::H + e];
-  }
-  const bool ok = tb[T::OK] != 0.0;
-  double b[P];
-#pragma unroll
-  for (int h = 0; h < P; ++h) b[h] = 0.0;
-  double rss = 0.0;
-  if (ok) {
-    double z[R], kz[R];
-#pragma unroll
-    for (int c = 0; c < R; ++c) {
-      double v = rb[c], k = 0.0;
-#pragma unroll
-      for (int j = 0; j < LB; ++j) {
-        v -= X[j][c] * zt[j];
-        if (R > 1) k += KT[R > 1 ? j : 0][c] * zt[j];
-      }
-      z[c] = v;
-      kz[c] = k;
-    }
-#pragma unroll
-    for (int i = 0; i < R; ++i) {   // U_S' z = num (diagonal reciprocals in the table)
-      double v = z[i];
-#pragma unroll
-      for (int k = 0; k < i; ++k) v -= us[upk(k, i)] * z[k];
-      z[i] = v * us[upk(i, i)];
-    }
-    double zz = 0.0;
-#pragma unroll
-    for (int c = 0; c < R; ++c) zz += z[c] * z[c];
-    double br[R];
-#pragma unroll
-    for (int i = R - 1; i >= 0; --i) {   // U_S b_r = z
-      double v = z[i];
-#pragma unroll
-      for (int k = i + 1; k < R; ++k) v -= us[upk(i, k)] * br[k];
-      br[i] = v * us[upk(i, i)];
-    }
-    rss = rss0 - zz;
-    if (R > 1) {   // misplaced-correction terms (zero for a pattern without missing rows)
-      double q = 0.0;
-#pragma unroll
-      for (int c = 0; c < R; ++c) q += 2.0 * kz[c] * br[c];
-#pragma unroll
-      for (int b2 = 0; b2 < R; ++b2)
-#pragma unroll
-        for (int a = 0; a <= b2; ++a) q -= h2[upk(a, b2)] * br[a] * br[b2];
-      rss += q;
-    }
-#pragma unroll
-    for (int c = 0; c < R; ++c) b[LB + c] = br[c];
-    if (d.need_bl || yzero || !(rss >= guard)) {   // reported, or the pair is re-evaluated
-#pragma unroll
-      for (int j = 0; j < LB; ++j)
-        if (j < l) {
-          double v = d.ay[(size_t)j * d.ldt + y];
-#pragma unroll
-          for (int c = 0; c < R; ++c) v -= tb[T::M + j * R + c] * br[c];
-          b[j] = v;
-        }
-    }
-  }
-  finish_pair<R, LB>(d, tk, y, s, ok, b, rb, rss);
-}
-
 // Explicit residual pass for the pairs the epilogue handed over: one wave
 // per pair.
 template <int R, int LB>
@@ -1006,17 +531,6 @@ __global__ __launch_bounds__(256) void k_subset_sbr(DevScan d, double* out) {
 
 // ---------------------------------------------------------------------------
 // launchers
-// (R, LB) pairs with compiled kernels: R=1: LB 8/16/24/32; R=2: 8/16; R=3,4: 8.
-#define OA_DISPATCH_RLB(R_, LB_, FN)          \
-  if (d.r == 1 && lb == 8) { FN(1, 8); }      \
-  else if (d.r == 1 && lb == 16) { FN(1, 16); } \
-  else if (d.r == 1 && lb == 24) { FN(1, 24); } \
-  else if (d.r == 1) { FN(1, 32); }           \
-  else if (d.r == 2 && lb == 8) { FN(2, 8); } \
-  else if (d.r == 2) { FN(2, 16); }           \
-  else if (d.r == 3) { FN(3, 8); }            \
-  else { FN(4, 8); }
-
 void launch_snp_gram(const DevScan& d, int prec, int lb, hipStream_t st) {
   dim3 grid((d.mc + 63) / 64), block(64);
 #define OA_SG(R_, LB_)                                                                     \
@@ -1064,30 +578,6 @@ void launch_epilogue(const DevScan& d, int prec, int lb, hipStream_t st) {
   else hipLaunchKernelGGL((k_epilogue<R_, LB_, float>), grid, block, 0, st, d, n_q);
   OA_DISPATCH_RLB(0, 0, OA_EP)
 #undef OA_EP
-}
-
-void launch_patsnp(const DevScan& d, int lb, hipStream_t st) {
-  const long long nthr = (long long)d.mc * d.G;
-  dim3 grid((unsigned)((nthr + 255) / 256)), block(256);
-#define OA_PS(R_, LB_) hipLaunchKernelGGL((k_patsnp<R_, LB_>), grid, block, 0, st, d);
-  OA_DISPATCH_RLB(0, 0, OA_PS)
-#undef OA_PS
-}
-
-int patsnp_width(int r, int lb) {
-  const int rr = r * (r + 1) / 2;
-  const int ok = lb * r + (r > 1 ? lb * r : 0) + 2 * rr + lb * r;
-  return (ok + 2) & ~1;   // PatSnp<R, LB>::W
-}
-
-void launch_epilogue_pst(const DevScan& d, int prec, int lb, hipStream_t st) {
-  const int n_tb = (d.t + 63) / 64, n_sq = (d.mc + 3) / 4;
-  dim3 grid((unsigned)((long long)n_tb * ((n_sq + 7) / 8) * 8)), block(256);
-#define OA_EQ(R_, LB_)                                                                                    \
-  if (prec == PREC_FP64) hipLaunchKernelGGL((k_epilogue_pst<R_, LB_, double>), grid, block, 0, st, d, n_tb); \
-  else hipLaunchKernelGGL((k_epilogue_pst<R_, LB_, float>), grid, block, 0, st, d, n_tb);
-  OA_DISPATCH_RLB(0, 0, OA_EQ)
-#undef OA_EQ
 }
 
 void launch_fallback(const DevScan& d, int lb, hipStream_t st) {
