@@ -12,7 +12,7 @@ struct FallbackEntry {
   int s;        // SNP within the chunk
   int flags;    // bit0: b is the untouched right-hand side (failed Cholesky)
   int pad;
-  double b[PMAX];
+  double b[PMAX];  // coefficients in the order h: covariates 0..l-1, then the SNP terms
 };
 
 enum {

@@ -182,8 +182,13 @@ __device__ __forceinline__ void finish_pair(const DevScan& d, const TraitK& tk, 
       f->y = y;
       f->s = s;
       f->flags = ok ? 0 : 1;
+      // coefficients in the reference's order h (covariates 0..l-1, then
+      // the SNP terms): the fallback kernel may be compiled for another LB
 #pragma unroll
-      for (int h = 0; h < LB + R; ++h) f->b[h] = b[h];
+      for (int h = 0; h < LB; ++h)
+        if (h < d.l) f->b[h] = b[h];
+#pragma unroll
+      for (int c = 0; c < R; ++c) f->b[d.l + c] = b[LB + c];
       return;
     }
     rss = explicit_rss_thread<R, LB>(d, y, s, b);

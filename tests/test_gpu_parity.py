@@ -88,17 +88,21 @@ def test_chunked_shared_patterns(oracle_mod):
         assert g.stats["n_chunks"] == 6
 
 
-def test_observed_zero_and_near_perfect_fit(oracle_mod):
+@pytest.mark.parametrize("miss", ["mcar", "groups"])
+def test_observed_zero_and_near_perfect_fit(miss, oracle_mod):
     """Observed phenotype values of exactly 0 (the reference's RSS skips
-    them) and near-perfect fits both take the explicit residual pass."""
-    data = synth.make(n=300, l=3, m=12, t=16, miss="mcar", miss_rate=0.05, planted=0, seed=5)
+    them) and near-perfect fits both take the explicit residual pass, from
+    the per-pair epilogue (independent patterns) and from the shared-pattern
+    table epilogue (groups)."""
+    data = synth.make(n=300, l=3, m=12, t=16, miss=miss, miss_rate=0.05, pattern_groups=2,
+                      pattern_rate=0.05, planted=0, seed=5)
     Y = data.Y.clone()
     Y[0, :40] = 0.0                      # observed zeros
     g0 = data.G[3].double()
     Y[1] = 0.5 * g0 + 1e-6 * torch.randn(data.n, dtype=torch.float64)  # near-perfect fit on SNP 3
     Y[2, 5] = 0.0
     data.Y = Y
-    print("case zeros/perfect", flush=True)
+    print("case zeros/perfect", miss, flush=True)
     g = run_gpu(data, "fp64", store_all=True)
     o = run_oracle(oracle_mod, data, store_all=True)
     compare_fp64(g, o, rtol=1e-9)
