@@ -294,6 +294,40 @@ static void prep_r(const DevScan& d, int prec, void* Ahi, void* Alo, bool corr, 
   }
 }
 
+// The request server's compact rows when the panel has no missing genotype:
+// Xu[u][s] = G[s0 + s][u] for every individual u, the panel chunk
+// transposed (exact in the panel's own type).  64 x 64 tiles through LDS, so
+// the reads along individuals and the writes along SNPs are both 128-byte
+// (float16) or 256-byte (float32) runs per wave, where the operand pass
+// would write 8-byte pieces to 4 rows per lane.
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel_rows(const T* __restrict__ G, long long ldg, int s0, int mc, int n,
+                                                    T* __restrict__ Xu, long long ldxu) {
+  __shared__ T tile[64][65];
+  const int u0 = blockIdx.x * 64, sb = blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int i = ty; i < 64; i += 4) {   // SNP sb + i, individuals u0 + tx
+    const int sn = sb + i, u = u0 + tx;
+    tile[i][tx] = (sn < mc && u < n) ? G[(size_t)(s0 + sn) * ldg + u] : (T)0;
+  }
+  __syncthreads();
+  for (int i = ty; i < 64; i += 4) {   // individual u0 + i, SNPs sb + tx
+    const int u = u0 + i, sn = sb + tx;
+    if (u < n && sn < mc) Xu[(size_t)u * ldxu + sn] = tile[tx][i];
+  }
+}
+
+void launch_panel_rows(const void* G, int gdt, long long ldg, int s0, int mc, int n, void* Xu, long long ldxu,
+                       hipStream_t st) {
+  dim3 grid((n + 63) / 64, (mc + 63) / 64), block(256);
+  if (gdt == DT_F32)
+    hipLaunchKernelGGL(k_panel_rows<float>, grid, block, 0, st, static_cast<const float*>(G), ldg, s0, mc, n,
+                       static_cast<float*>(Xu), ldxu);
+  else
+    hipLaunchKernelGGL(k_panel_rows<_Float16>, grid, block, 0, st, static_cast<const _Float16*>(G), ldg, s0, mc, n,
+                       static_cast<_Float16*>(Xu), ldxu);
+}
+
 void launch_prep(const DevScan& d, int prec, void* Ahi, void* Alo, bool corr, hipStream_t st) {
   switch (d.r) {
     case 1: prep_r<1>(d, prec, Ahi, Alo, corr, st); break;

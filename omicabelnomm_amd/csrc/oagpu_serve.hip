@@ -304,7 +304,14 @@ int run_chunk(oagpu_server* S) {
   dp.Xu = S->Xu;
   dp.xu_dt = S->xu_dt;
   dp.ldxu = S->ldxu;
-  launch_prep(dp, prec, S->Ahi, S->Alo, any_missing, S->st);
+  if (any_missing && S->xu_dt != DT_F64) {
+    // no missing genotype in the panel: the compact rows are the chunk itself,
+    // transposed, written by a tiled kernel instead of the operand pass
+    launch_prep(dp, prec, S->Ahi, S->Alo, false, S->st);
+    launch_panel_rows(P->G, S->xu_dt, P->ldg, s0, mcc, P->n, S->Xu, S->ldxu, S->st);
+  } else {
+    launch_prep(dp, prec, S->Ahi, S->Alo, any_missing, S->st);
+  }
 
   // one contraction for every active request's columns
   const bool split = prec == OAGPU_BF16X3;
