@@ -9,7 +9,9 @@ plain scan.
 - a request's answer is the same alone and in a mix of concurrent requests
   (bit-identical in FP64; split-BF16 within 1e-4 SE on beta, as the mixed
   path is held to against FP64);
-- a budget that admits one request at a time still answers every request.
+- a budget that admits one request at a time still answers every request;
+- a panel with missing genotypes (mean-imputed, compact rows from the
+  operand pass) answers as the plain scan does.
 """
 import numpy as np
 import pytest
@@ -147,4 +149,24 @@ def test_budget_admits_one_at_a_time():
         assert_hits_equal(a.hits, scan_alone(G, Y, XL, None, "fp64", **THR), 1e-12)
     # one at a time: each finished before the next started, so latencies grow
     assert answers[0].latency_ms < answers[2].latency_ms
+    panel.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+def test_panel_with_missing_genotypes(precision):
+    """A panel holding missing genotypes (imputed with the panel-wide mean;
+    the compact missing-row values are then written, imputed, by the operand
+    pass in FP64): every answer equals the plain scan of the same panel."""
+    d = synth.make(n=400, l=2, m=150, t=2, miss="none", geno_nan=0.02, seed=45)
+    G = d.G.contiguous()
+    n = G.shape[1]
+    assert torch.isnan(G).any()
+    reqs = [request_data(n, 12, 3, "mcar", 31), request_data(n, 20, 2, "groups", 32)]
+    print("case nan panel", precision, flush=True)
+    panel = Panel(G.to("cuda"))
+    with Server(panel, precision, chunk_snps=64) as srv:
+        ids = [srv.submit(Y, XL, store_all=True, **THR) for Y, XL in reqs]
+        answers = [srv.wait(i) for i in ids]
+    for (Y, XL), a in zip(reqs, answers):
+        assert_hits_equal(a.hits, scan_alone(G, Y, XL, None, precision, **THR), 1e-12 if precision == "fp64" else 1e-4)
     panel.close()
