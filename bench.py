@@ -31,6 +31,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "SNPxtrait tests/sec, mixed vs FP64 at max |Δ-log10p|≤1e-3; p50/p99 latency"
 PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}   # TFLOP/s, MI355X dense MFMA
+# MFMA products per algorithmic multiply-add of the contraction
+MFMA_PRODUCTS = {"bf16x3": 3, "fp32": 1, "fp64": 1}
 
 
 def parse():
@@ -410,7 +412,11 @@ def main():
         "roofline": {"bound": "mfma", "achieved": gemm_tflops, "peak": peak, "unit": "TFLOP/s",
                      "frac": gemm_tflops / peak, "traffic": None if over else measured_traffic(variant, args.config),
                      "kernel": variant,
+                     "mfma_products": MFMA_PRODUCTS[args.precision],
+                     "hardware_frac": gemm_tflops * MFMA_PRODUCTS[args.precision] / peak,
                      "note": "algorithmic 2*n*r*m*(t+l) flops / HIP-event time of the contraction; "
+                             "hardware_frac counts the MFMA products issued per algorithmic multiply-add "
+                             "(3 for split-BF16: lo*hi, hi*lo, hi*hi); "
                              "traffic = HBM bytes per contraction (rocprofv3 FETCH_SIZE, WRITE_SIZE, "
                              "copy-calibrated) when a profile of this kernel is committed"},
         "cpu_baseline": cpu,
