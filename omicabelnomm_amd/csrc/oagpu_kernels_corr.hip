@@ -10,23 +10,24 @@ __device__ __forceinline__ double widen(double v) { return v; }
 __device__ __forceinline__ double widen(float v) { return (double)v; }
 __device__ __forceinline__ double widen(_Float16 v) { return (double)(float)v; }
 
-// generate_correction_missings_Str (src/Algorithm.cpp:1260-1315) for every
-// pattern: C_lr = sum over the pattern's missing rows of xl x', C_rr of x x'.
-// The genotype values come from the compact u-major rows of U written by the
-// operand pass.  One wave per pattern, one lane per SNP: the pattern's row
-// positions and covariate rows are wave-uniform, and the 64 SNPs' values at
-// one row are contiguous.  Blocks are numbered so that the pattern groups of
-// one 64-SNP tile run back to back on one XCD and share its L2.
-// SPL consecutive SNPs per lane (one vector load per row); two when a SNP
-// has one column and there are at most 16 covariates, so every covariate
-// value read from LDS feeds twice the multiply-adds without raising the
-// register count past 5 waves per SIMD.
+// Two consecutive SNPs per lane when a SNP has one column and there are at
+// most 16 covariates: every covariate value read from LDS then feeds twice
+// the multiply-adds without raising the register count past 5 waves per SIMD.
 template <int R, int LB>
 constexpr int corr_spl() { return (R == 1 && LB <= 16) ? 2 : 1; }
 
 template <typename XT, int N> struct XVec { typedef __attribute__((ext_vector_type(N))) XT T; };
 template <typename XT> struct XVec<XT, 1> { typedef XT T; };
 
+// generate_correction_missings_Str (src/Algorithm.cpp:1260-1315) for every
+// pattern: C_lr = sum over the pattern's missing rows of xl x', C_rr of x x'.
+// The genotype values come from the compact u-major rows of U written by the
+// operand pass.  One wave per pattern, corr_spl() consecutive SNPs per lane
+// (one vector load per row): the pattern's row positions and covariate rows
+// are wave-uniform, and the wave's SNPs at one row are contiguous.  Blocks are
+// numbered so that the pattern groups of one SNP tile run back to back on one
+// XCD and share its L2.  LB is the covariate count the kernel is compiled
+// for: the exact l, or a bucket whose entries past l are zero.
 template <int R, int LB, typename XT>
 __global__ __launch_bounds__(256) void k_corr(DevScan d, int gq_count) {
   constexpr int SPL = corr_spl<R, LB>();

@@ -1,18 +1,17 @@
-// SNP-side kernels of one chunk.  They replace, per SNP block:
-//   replace_nans + replace_nans_avgs on XR .. src/Algorithm.cpp:618-622, src/Utility.cpp:103-127
-//   interaction synthesis .................... src/AIOwrapper.cpp:767-840 (fused into the loads)
-//   Sbr = XR_s'XR_s, build_S, potrf/potri .... src/Algorithm.cpp:646-671
-//   generate_correction_missings_Str ......... src/Algorithm.cpp:1260-1315
-//   per-pair build_S/correct_missings_S/sposv  src/Algorithm.cpp:714-792, 1370-1399
-//   hpc_SSY (RSS) ............................ src/Algorithm.cpp:896-1024
-//   hpc_statistics ........................... src/Algorithm.cpp:1034-1214
-// Everything here is FP64.  The pair solve uses the block structure of
-// S_pair = [[A_g, B],[B', D]]: A_g (covariates, per missing pattern) is
-// factored once per pattern, each pair factors only its r x r Schur
-// complement, and RSS is the downdate RSS0(trait) - |z_r|^2 of the
-// covariate-only residual, plus the terms the reference's misplaced
-// corrections add.  In exact arithmetic this is the reference's
-// full-matrix Cholesky solve (same pivots, same failure rule).
+// SNP-side kernels of one chunk: the per-SNP Gram pieces Str, var_x
+// (build_S + potrf/potri on the unmasked S_full, src/Algorithm.cpp:646-671),
+// the explicit residual pass for the pairs the epilogues hand over (hpc_SSY,
+// src/Algorithm.cpp:896-1024), the subset Sbr of the request server and the
+// counter resets.  The other stages of the chunk live beside this file:
+//   oagpu_kernels_prep.hip  replace_nans(_avgs) on XR, interaction columns,
+//                           Sbr = XR_s'XR_s (src/Algorithm.cpp:618-622,
+//                           src/Utility.cpp:103-127, src/AIOwrapper.cpp:767-840)
+//   oagpu_kernels_corr.hip  generate_correction_missings_Str
+//                           (src/Algorithm.cpp:1260-1315)
+//   oagpu_kernels_pst.hip,  per-pair build_S/correct_missings_S/sposv, RSS and
+//   oagpu_kernels_epi.hip   hpc_statistics (src/Algorithm.cpp:714-792,
+//                           1034-1214, 1370-1399)
+// Everything here is FP64.
 #include "oagpu_kernels.h"
 #include "oagpu_geno.h"
 #include "oagpu_pair.h"
