@@ -1,6 +1,16 @@
 // Pair statistics shared by the pair kernels (oagpu_kernels_snp.hip,
-// oagpu_kernels_pst.hip): hpc_statistics, the explicit residual, the hand-over
-// to the fallback pass, and the (R, LB) dispatch of the compiled kernels.
+// oagpu_kernels_pst.hip, oagpu_kernels_epi.hip): hpc_statistics, the explicit
+// residual, the hand-over to the fallback pass, and the (R, LB) dispatch of
+// the compiled kernels.
+//
+// The pair solve (all FP64) uses the block structure of
+// S_pair = [[A_g, B],[B', D]]: A_g (covariates, per missing pattern) is
+// factored once per pattern, each pair factors only its r x r Schur
+// complement, and RSS is the downdate RSS0(trait) - |z_r|^2 of the
+// covariate-only residual, plus the terms the reference's misplaced
+// corrections add.  In exact arithmetic this is the reference's full-matrix
+// Cholesky solve (same pivots, same failure rule; src/Algorithm.cpp:714-792,
+// 1370-1399).
 #pragma once
 #include "oagpu_kernels.h"
 #include "oagpu_geno.h"
