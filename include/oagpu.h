@@ -141,6 +141,23 @@ int oagpu_set_interactions(oagpu_ctx *ctx, const double *E, long long ld,
 int oagpu_load_traits(oagpu_ctx *ctx, const void *Y, int dtype, int where,
                       long long ld, int t, int trait_offset);
 
+/* Dosage model (src/AIOwrapper.cpp:256-340 sets it up, 1208-1297 applies it
+ * to every genotype block): the genotype file holds file_r columns per SNP
+ * (e.g. P(AA), P(AB), P(BB)) and only the first n_used of them are read.
+ * OAGPU_DOSAGE_SUM makes one column per SNP, sum_i f[i] * G[:, s*file_r+i]
+ * (the reference's additive, dominant, recessive and -y models, r = 1);
+ * OAGPU_DOSAGE_SCALE makes n_used columns, f[i] * G[:, s*file_r+i] (the -z
+ * linear model).  The columns are formed on the device, in FP64, before
+ * NaN imputation, as the reference combines before replace_nans_avgs.
+ * Afterwards oagpu_load_genotypes takes blocks of whole file SNPs (n_src a
+ * multiple of file_r) and its column descriptors index the combined
+ * columns.  mode OAGPU_DOSAGE_NONE switches the model off.  At most
+ * OAGPU_MAX_DOSAGE_FACTORS factors. */
+#define OAGPU_MAX_DOSAGE_FACTORS 16
+enum { OAGPU_DOSAGE_NONE = 0, OAGPU_DOSAGE_SUM = 1, OAGPU_DOSAGE_SCALE = 2 };
+int oagpu_set_dosage_model(oagpu_ctx *ctx, int mode, int file_r, int n_used,
+                           const double *factors);
+
 /* One block of m SNPs.  G holds the source genotype columns (n rows,
  * leading dimension ld, NaN = missing, imputed with the column mean).
  * Output column c of SNP s (c < r) is G[:, col_src[s*r+c]], times

@@ -33,6 +33,11 @@ typedef struct omicabelnomm_settings {
     int precision;               /* OAGPU_FP64 / OAGPU_FP32 / OAGPU_BF16X3     */
     int device;
     int verbose;                 /* 1: the reference's console messages       */
+    int dosages;                 /* -j/-k/-l/-z/-y given (Settings::dosages)   */
+    int model;                   /* Settings::model: -1 none, 0 additive,
+                                    1 dominant, 2 recessive, 3 linear from
+                                    fname_dosages, 4 additive from it         */
+    const char *fname_dosages;   /* factor list of models 3 and 4            */
 } omicabelnomm_settings;
 
 /* Outputs (src/Definitions.h:222-257): the fields this driver fills. */

@@ -80,8 +80,61 @@ int apply_exclusion_file(const std::string& fname, int n, std::vector<uint8_t>& 
   return excluded;
 }
 
+// read_dosages (src/AIOwrapper.cpp:2236-2256): `expected` factors read with
+// operator>>.  The reference also counts a read that fails (at the end of
+// the file after a trailing blank, or on a token that is not a number) and
+// leaves that factor uninitialised; here the list must hold `expected`
+// numbers.
+std::vector<double> read_dosage_factors(const std::string& fname, int expected) {
+  std::ifstream in(fname);
+  if (!in) throw Fail(1, "Error reading dosages file.");
+  std::vector<double> f;
+  int i = 0;
+  for (; i < expected; ++i) {
+    float v = 0.0f;
+    if (!(in >> v)) break;
+    f.push_back(v);
+  }
+  if (i != expected) {
+    std::ostringstream m;
+    m << "The number of factors for the dosage model! does not coincide with the source data #facctors:" << i
+      << ", expected:" << expected;
+    throw Fail(1, m.str());
+  }
+  return f;
+}
+
+// Dosage model set-up (src/AIOwrapper.cpp:256-340).  Fills the factors and
+// whether they are summed into one column per SNP.  The recessive model is
+// the one its message states, 1*AA (the reference reads an uninitialised
+// second factor for three columns per SNP and no column for two).
+void dosage_setup(const omicabelnomm_settings& s, int file_r, std::vector<double>& f, bool& summed) {
+  static const char* names[] = {"Additive", "Dominant", "Recessive"};
+  switch (s.model) {
+    case 0: case 1: case 2:
+      if (file_r != 2 && file_r != 3)
+        throw Fail(1, std::string("The amount of columns per Independent Variable (--ngpred) is not 2 or 3 for a "
+                                  "valid ") + names[s.model] + " Model!");
+      f = s.model == 0 ? std::vector<double>{2.0, 1.0}
+                       : (s.model == 1 ? std::vector<double>{1.0, 1.0} : std::vector<double>{1.0});
+      summed = true;
+      return;
+    case 3: case 4:
+      if (file_r > OAGPU_MAX_DOSAGE_FACTORS)
+        throw Fail(1, "dosage model: at most " + std::to_string(OAGPU_MAX_DOSAGE_FACTORS) + " columns per SNP");
+      f = read_dosage_factors(str_or(s.fname_dosages), file_r);
+      summed = s.model == 4;
+      return;
+    default:
+      throw Fail(1, "Requested dosages model wihtout a valid model!");
+  }
+}
+
 struct Plan {
   int n_file = 0, n = 0, realN = 0, l = 0, r_file = 1, r = 1, t = 0, m_file = 0, m = 0;
+  int r_model = 1;                     // genotype columns per file SNP after the dosage model
+  bool dos = false, dos_sum = false;
+  std::vector<double> dos_f;
   int tb = 0, mb = 0;                  // reference block sizes (output grouping)
   bool inter = false, multi = false, keep = false;
   int num_inter = 0, ini_inter = 0, file_block = 0;   // multi-set: file SNPs per block
@@ -101,7 +154,7 @@ void descriptors(const Plan& P, int f0, int nf, std::vector<int32_t>& src, std::
     nv = nf;
     for (int s = 0; s < nf; ++s)
       for (int c = 0; c < P.r; ++c) {
-        src.push_back(s * P.r_file + c);
+        src.push_back(s * P.r_model + c);
         env.push_back(-1);
       }
     return;
@@ -160,6 +213,9 @@ extern "C" void omicabelnomm_apply_defaults(omicabelnomm_settings* s) {
   s->precision = OAGPU_FP64;
   s->device = 0;
   s->verbose = 1;
+  s->dosages = 0;
+  s->model = -1;
+  s->fname_dosages = "";
 }
 
 static int solve_impl(const omicabelnomm_settings* sin, omicabelnomm_outputs* out) {
@@ -224,14 +280,25 @@ static int solve_impl(const omicabelnomm_settings* sin, omicabelnomm_outputs* ou
         }
     if (almiss && verbose) std::cout << "Excluding " << almiss << " from covariate missings" << std::endl;
   }
+  // Dosage model (src/AIOwrapper.cpp:33-38,256-340).
+  P.r_model = P.r_file;
+  if (s.dosages || s.model != -1) {
+    if (s.model == -1) throw Fail(1, "Requested dosages model wihtout a valid model!");
+    dosage_setup(s, P.r_file, P.dos_f, P.dos_sum);
+    P.dos = true;
+    P.r_model = P.dos_sum ? 1 : (int)P.dos_f.size();
+  }
   std::vector<double> e_all;
   Fvi ifvi;
   if (s.use_interactions) {
     P.inter = true;
     P.multi = s.use_multiple_interaction_sets != 0;
     P.keep = s.keep_depVar != 0;
-    if (P.r_file != 1)
-      throw Fail(1, "Cannot use Interactions with --ngpred set to: " + std::to_string(P.r_file) +
+    if (P.r_model != 1)   // src/AIOwrapper.cpp:352-366
+      throw Fail(1, std::string(P.dos ? "Cannot use Interactions with the current dosage model! #Of factors can be "
+                                        "at most 1!\n"
+                                      : "") +
+                        "Cannot use Interactions with --ngpred set to: " + std::to_string(P.r_file) +
                         ". Required is 1!");
     const std::string fI = str_or(s.fname_interactions);
     ifvi = read_fvi(fI + ".fvi");
@@ -262,10 +329,12 @@ static int solve_impl(const omicabelnomm_settings* sin, omicabelnomm_outputs* ou
 
   // Model shape and AR names (src/AIOwrapper.cpp:421-446, 563-657).
   const int mb_file = std::min(P.m_file, 500);
+  // Under a summed dosage model a SNP is named by its first column.
   std::vector<std::string> file_names;
-  for (int i = 0; i < P.m_file * P.r_file; ++i) file_names.push_back(arfvi.var_name(i));
+  for (int i = 0; i < P.m_file * P.r_file; i += (P.dos && P.dos_sum) ? P.r_file : 1)
+    file_names.push_back(arfvi.var_name(i));
   if (!P.inter) {
-    P.r = P.r_file;
+    P.r = P.r_model;
     P.m = P.m_file;
     P.mb = mb_file;
     P.ar_names = file_names;
@@ -308,7 +377,8 @@ static int solve_impl(const omicabelnomm_settings* sin, omicabelnomm_outputs* ou
   if (s.storeBin) {
     std::vector<const char*> aln, arn, yn;
     for (int i = 1; i < P.l; ++i) aln.push_back(alfvi.var_name_raw(i));
-    for (int i = 0; i < P.m_file * P.r_file; ++i) arn.push_back(arfvi.var_name_raw(i));
+    for (int i = 0; i < P.m_file * P.r_file; i += (P.dos && P.dos_sum) ? P.r_file : 1)
+      arn.push_back(arfvi.var_name_raw(i));
     for (int i = 0; i < P.t; ++i) yn.push_back(yfvi.var_name_raw(i));
     write_ibin(fOut, P.n, P.l, P.r, P.t, P.m, arfvi.namelength, s.disp_cov, s.storePInd, aln, arn, yn);
   }
@@ -357,6 +427,11 @@ static int solve_impl(const omicabelnomm_settings* sin, omicabelnomm_outputs* ou
   }
   rc = oagpu_set_covariates(ctx, XL.data(), P.n);
   if (rc != OAGPU_OK) throw Fail(1, oagpu_last_error(ctx));
+  if (P.dos) {
+    rc = oagpu_set_dosage_model(ctx, P.dos_sum ? OAGPU_DOSAGE_SUM : OAGPU_DOSAGE_SCALE, P.r_file,
+                                (int)P.dos_f.size(), P.dos_f.data());
+    if (rc != OAGPU_OK) throw Fail(1, oagpu_last_error(ctx));
+  }
   if (P.inter) {
     std::vector<double> E((size_t)P.n * P.num_inter);
     for (int h = 0; h < P.num_inter; ++h) {

@@ -5,7 +5,9 @@
 //   -v/--simpleinter and -w/--multinter take the interaction file path (the
 //     reference assigns atof(optarg) to the path, src/main.cpp:370,381);
 //   -u/--keepinter is a plain flag (the reference requires an argument);
-//   -j/-k/-l/-z/-y (dosage models) are reported as not supported;
+//   -z/--mylinear and -y/--myaddit take the factor file path (the reference
+//     never stores it, src/main.cpp:341-358, so its read fails);
+//   -l/--recessive is the model its message states, 1*AA;
 //   -m/--mem is accepted and ignored (the reference aborts);
 //   --precision fp64|fp32|bf16x3 and --device N select the GPU path.
 #include <getopt.h>
@@ -23,7 +25,8 @@
 static const char* kUsage =
     "usage: omicabelnomm -c <path/fname> --geno <path/fname> -p <path/fname> -o <path/fname> "
     "-x <path/fname> -n <#SNPcols> -t <#CPUs> -d <0.0~1.0> -r <-10.0~1.0> -b -s <0.0~1.0> "
-    "-e <-10.0~1.0> -i -f [--precision fp64|fp32|bf16x3] [--device N]";
+    "-e <-10.0~1.0> -i -f [-j|-k|-l|-z <factors>|-y <factors>] [--precision fp64|fp32|bf16x3] "
+    "[--device N]";
 
 static std::string strip_ext(const char* arg) {
   std::string s(arg);
@@ -34,7 +37,7 @@ static std::string strip_ext(const char* arg) {
 int main(int argc, char* argv[]) {
   omicabelnomm_settings s;
   omicabelnomm_apply_defaults(&s);
-  std::string fY, fAR, fAL, fOut, fExcl, fInter;
+  std::string fY, fAR, fAL, fOut, fExcl, fInter, fDos;
   bool phe = false, snp = false, cov = false, bout = false;
   static struct option opts[] = {
       {"phe", required_argument, 0, 'p'},       {"geno", required_argument, 0, 'g'},
@@ -136,10 +139,30 @@ int main(int argc, char* argv[]) {
         s.keep_depVar = 1;
         std::cout << "-u Keeping independent variable for interaction analysis " << std::endl;
         break;
-      case 'j': case 'k': case 'l': case 'z': case 'y':
-        std::cerr << "\nerror with argument parameter " << (char)c
-                  << ": dosage models are not supported by this build" << std::endl;
-        return 1;
+      case 'j':   // src/main.cpp:314-358
+        s.model = 0;
+        s.dosages = 1;
+        std::cout << "-j Using Additive Model with (2*AA, 1*AB, 0*BB) effects" << std::endl;
+        break;
+      case 'k':
+        s.model = 1;
+        s.dosages = 1;
+        std::cout << "-k Using Dominant Model with (1*AA, 1*AB, 0*BB) effects" << std::endl;
+        break;
+      case 'l':
+        s.model = 2;
+        s.dosages = 1;
+        std::cout << "-l Using Recessive Model with (1*AA, 0*AB, 0*BB) effects" << std::endl;
+        break;
+      case 'z':
+      case 'y':
+        s.model = c == 'z' ? 3 : 4;
+        s.dosages = 1;
+        fDos = optarg;
+        std::cout << (c == 'z' ? "-z Using Custom Linear Model with parameters read from the file "
+                               : "-y Using Custom Additive Model with parameters read from the file ")
+                  << fDos << std::endl;
+        break;
       case 1000:
         if (!std::strcmp(optarg, "fp64")) s.precision = OAGPU_FP64;
         else if (!std::strcmp(optarg, "fp32")) s.precision = OAGPU_FP32;
@@ -175,6 +198,7 @@ int main(int argc, char* argv[]) {
   s.fnameOutFiles = fOut.c_str();
   s.fname_excludelist = fExcl.c_str();
   s.fname_interactions = fInter.c_str();
+  s.fname_dosages = fDos.c_str();
   std::cout << "omicabelnomm (MI355X)" << std::endl;
   omicabelnomm_outputs out;
   std::memset(&out, 0, sizeof(out));

@@ -214,6 +214,7 @@ extern "C" void oagpu_destroy(oagpu_ctx* c) {
   if (c->cst) (void)hipStreamSynchronize(c->cst);
   for (int i = 0; i < 2; ++i) {
     dfree(c->G_slot[i]);
+    dfree(c->G_dos[i]);
     dfree(c->d_src[i]);
     dfree(c->d_env[i]);
     if (c->up_ev[i]) (void)hipEventDestroy(c->up_ev[i]);
@@ -551,10 +552,17 @@ extern "C" int oagpu_load_genotypes(oagpu_ctx* c, const void* G, int dtype, int 
     return fail(c, OAGPU_ERR_ARG, "bad genotype block");
   HIPCK(c, hipSetDevice(c->prm.device));
   const int r = c->r;
-  if (!col_src && (long long)m * r > n_src) return fail(c, OAGPU_ERR_ARG, "m*r exceeds the source columns");
+  // Under a dosage model the descriptors index the combined columns.
+  int n_col = n_src;
+  if (c->dos_on) {
+    if (n_src % c->dos.file_r != 0)
+      return fail(c, OAGPU_ERR_ARG, "dosage model: the block is not a whole number of file SNPs");
+    n_col = (n_src / c->dos.file_r) * (c->dos.sum ? 1 : c->dos.n_used);
+  }
+  if (!col_src && (long long)m * r > n_col) return fail(c, OAGPU_ERR_ARG, "m*r exceeds the source columns");
   if (col_src)
     for (long long i = 0; i < (long long)m * r; ++i)
-      if (col_src[i] < 0 || col_src[i] >= n_src) return fail(c, OAGPU_ERR_ARG, "column descriptor out of range");
+      if (col_src[i] < 0 || col_src[i] >= n_col) return fail(c, OAGPU_ERR_ARG, "column descriptor out of range");
   if (col_env) {
     for (long long i = 0; i < (long long)m * r; ++i)
       if (col_env[i] >= c->n_env) return fail(c, OAGPU_ERR_ARG, "interaction column out of range");
@@ -577,6 +585,21 @@ extern "C" int oagpu_load_genotypes(oagpu_ctx* c, const void* G, int dtype, int 
     c->Gm = c->G_slot[slot];
   } else {
     c->Gm = G;
+  }
+  if (c->dos_on) {   // combine on the copy stream; the scan waits for up_ev
+    const long long ldo = ((long long)c->n + 3) & ~3ll;
+    const size_t bytes = sizeof(double) * (size_t)ldo * n_col;
+    if (c->G_dos_cap[slot] < bytes) {
+      HIPCK(c, hipStreamSynchronize(c->cst));
+      HIPCK(c, grow(&c->G_dos[slot], &c->G_dos_cap[slot], bytes));
+    }
+    launch_dosage(c->Gm, dtype, ld, c->n, n_src / c->dos.file_r, c->dos, static_cast<double*>(c->G_dos[slot]), ldo,
+                  c->cst);
+    HIPCK(c, hipGetLastError());
+    c->Gm = c->G_dos[slot];
+    c->gdt = OAGPU_F64;
+    c->ldg = ldo;
+    c->n_src = n_col;
   }
   const size_t nd = (size_t)m * r;
   if (col_src || col_env) {
@@ -607,6 +630,26 @@ extern "C" int oagpu_load_genotypes(oagpu_ctx* c, const void* G, int dtype, int 
   c->gslot = slot;
   c->upload_pending = true;
   c->have_geno = true;
+  return OAGPU_OK;
+}
+
+extern "C" int oagpu_set_dosage_model(oagpu_ctx* c, int mode, int file_r, int n_used, const double* factors) {
+  if (!c) return OAGPU_ERR_ARG;
+  if (mode == OAGPU_DOSAGE_NONE) {
+    c->dos_on = false;
+    return OAGPU_OK;
+  }
+  if (mode != OAGPU_DOSAGE_SUM && mode != OAGPU_DOSAGE_SCALE)
+    return fail(c, OAGPU_ERR_ARG, "unknown dosage model mode");
+  if (!factors || file_r < 1 || n_used < 1 || n_used > file_r || n_used > OAGPU_MAX_DOSAGE_FACTORS)
+    return fail(c, OAGPU_ERR_ARG, "dosage model: need 1 <= n_used <= file_r and n_used <= 16 factors");
+  static_assert(sizeof(c->dos.f) / sizeof(double) == OAGPU_MAX_DOSAGE_FACTORS, "factor capacity");
+  c->dos = DosageModel{};
+  c->dos.file_r = file_r;
+  c->dos.n_used = n_used;
+  c->dos.sum = mode == OAGPU_DOSAGE_SUM;
+  for (int i = 0; i < n_used; ++i) c->dos.f[i] = factors[i];
+  c->dos_on = true;
   return OAGPU_OK;
 }
 

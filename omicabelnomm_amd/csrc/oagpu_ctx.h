@@ -85,6 +85,11 @@ struct oagpu_ctx {
   int gslot = 1;
   bool upload_pending = false;
   int *col_src = nullptr, *col_env = nullptr;
+  // dosage model: the combined FP64 columns of each slot's block
+  bool dos_on = false;
+  oa::DosageModel dos{};
+  void* G_dos[2] = {nullptr, nullptr};
+  size_t G_dos_cap[2] = {0, 0};
 
   // a scan in flight (oagpu_scan_async .. oagpu_scan_wait)
   bool pending = false;

@@ -28,6 +28,16 @@ void launch_prep(const DevScan& d, int prec, void* Ahi, void* Alo, bool corr, hi
 // Xu[u][s] = G[(s0 + s) * ldg + u] for u < n, s < mc (gdt DT_F32 or DT_F16)
 void launch_panel_rows(const void* G, int gdt, long long ldg, int s0, int mc, int n, void* Xu, long long ldxu,
                        hipStream_t st);
+// Dosage model: file SNP s has file_r columns of which the first n_used are
+// read; sum != 0 writes one column per SNP, sum_i f[i] * G[:, s*file_r+i],
+// otherwise n_used columns f[i] * G[:, s*file_r+i].  FP64 out (n rows,
+// leading dimension ldo), NaN kept.
+struct DosageModel {
+  int file_r, n_used, sum;
+  double f[16];
+};
+void launch_dosage(const void* G, int gdt, long long ldg, int n, int n_snp, const DosageModel& dm, double* out,
+                   long long ldo, hipStream_t st);
 void launch_snp_gram(const DevScan& d, int prec, int lb, hipStream_t st);
 void launch_corr(const DevScan& d, int lb, hipStream_t st);
 // one thread per pair, reading the corrections from d.Ccorr

@@ -328,6 +328,50 @@ void launch_panel_rows(const void* G, int gdt, long long ldg, int s0, int mc, in
                        static_cast<_Float16*>(Xu), ldxu);
 }
 
+// Dosage model columns, formed once per genotype block before the scan
+// (src/AIOwrapper.cpp:1258-1290: an sgemm of the n_used probability
+// columns with the factor vector, or each column times its factor).  One
+// thread per output element, individuals fastest, so reads and writes are
+// contiguous runs; HBM-bound, n_used reads and one FP64 write per element.
+template <typename GT>
+__global__ __launch_bounds__(256) void k_dosage(const GT* __restrict__ G, long long ldg, int n, int n_snp,
+                                                DosageModel dm, double* __restrict__ out, long long ldo) {
+  const int per = dm.sum ? 1 : dm.n_used;
+  const long long total = (long long)n_snp * per * n;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+    const long long col = e / n;
+    const int k = (int)(e - col * n);
+    const long long s = col / per;
+    const int j = (int)(col - s * per);
+    const GT* g = G + (size_t)(s * dm.file_r) * ldg + k;
+    double v;
+    if (dm.sum) {
+      v = 0.0;
+      for (int i = 0; i < dm.n_used; ++i) v += dm.f[i] * (double)g[(size_t)i * ldg];
+    } else {
+      v = dm.f[j] * (double)g[(size_t)j * ldg];
+    }
+    out[(size_t)col * ldo + k] = v;
+  }
+}
+
+void launch_dosage(const void* G, int gdt, long long ldg, int n, int n_snp, const DosageModel& dm, double* out,
+                   long long ldo, hipStream_t st) {
+  const long long total = (long long)n_snp * (dm.sum ? 1 : dm.n_used) * n;
+  if (total <= 0) return;
+  const int blocks = (int)std::min<long long>((total + 255) / 256, 1 << 16);
+  if (gdt == DT_F64)
+    hipLaunchKernelGGL(k_dosage<double>, dim3(blocks), dim3(256), 0, st, static_cast<const double*>(G), ldg, n,
+                       n_snp, dm, out, ldo);
+  else if (gdt == DT_F32)
+    hipLaunchKernelGGL(k_dosage<float>, dim3(blocks), dim3(256), 0, st, static_cast<const float*>(G), ldg, n,
+                       n_snp, dm, out, ldo);
+  else
+    hipLaunchKernelGGL(k_dosage<_Float16>, dim3(blocks), dim3(256), 0, st, static_cast<const _Float16*>(G), ldg,
+                       n, n_snp, dm, out, ldo);
+}
+
 void launch_prep(const DevScan& d, int prec, void* Ahi, void* Alo, bool corr, hipStream_t st) {
   switch (d.r) {
     case 1: prep_r<1>(d, prec, Ahi, Alo, corr, st); break;

@@ -19,6 +19,7 @@ ERR_NUMERIC, ERR_UNSUPPORTED, ERR_COVARIATES, ERR_NO_DEVICE = -5, -6, -7, -8
 
 FP64, FP32, BF16X3 = 0, 1, 2
 F64, F32, F16 = 0, 1, 2
+DOSAGE_NONE, DOSAGE_SUM, DOSAGE_SCALE = 0, 1, 2
 HOST, DEVICE = 0, 1
 
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3}
@@ -84,8 +85,8 @@ class Request(ctypes.Structure):
 
 
 EXPORTS = ["oagpu_create", "oagpu_destroy", "oagpu_last_error", "oagpu_set_subset", "oagpu_set_covariates",
-           "oagpu_set_interactions", "oagpu_load_traits", "oagpu_load_genotypes", "oagpu_scan",
-           "oagpu_scan_async", "oagpu_scan_wait", "oagpu_host_alloc", "oagpu_host_free",
+           "oagpu_set_interactions", "oagpu_set_dosage_model", "oagpu_load_traits", "oagpu_load_genotypes",
+           "oagpu_scan", "oagpu_scan_async", "oagpu_scan_wait", "oagpu_host_alloc", "oagpu_host_free",
            "oagpu_last_timings", "oagpu_gemm_variant", "oagpu_tvalue",
            "oagpu_panel_create", "oagpu_panel_destroy", "oagpu_server_create", "oagpu_server_destroy",
            "oagpu_server_last_error", "oagpu_server_submit", "oagpu_server_poll", "oagpu_server_wait"]
@@ -111,6 +112,7 @@ def load():
     L.oagpu_set_subset.argtypes = [vp, vp]
     L.oagpu_set_covariates.argtypes = [vp, vp, ll]
     L.oagpu_set_interactions.argtypes = [vp, vp, ll, i]
+    L.oagpu_set_dosage_model.argtypes = [vp, i, i, i, vp]
     L.oagpu_load_traits.argtypes = [vp, vp, i, i, ll, i, i]
     L.oagpu_load_genotypes.argtypes = [vp, vp, i, i, ll, i, i, vp, vp, i]
     L.oagpu_scan.argtypes = [vp, ctypes.POINTER(Dense), vp, ll, ctypes.POINTER(Stats)]
@@ -133,8 +135,8 @@ def load():
     L.oagpu_server_poll.argtypes = [vp, ll]
     L.oagpu_server_wait.argtypes = [vp, ll, vp, ll, ctypes.POINTER(Stats), ctypes.POINTER(ctypes.c_double),
                                     ctypes.c_char_p, i]
-    for f in ["oagpu_set_subset", "oagpu_set_covariates", "oagpu_set_interactions", "oagpu_load_traits",
-              "oagpu_load_genotypes", "oagpu_scan", "oagpu_last_timings", "oagpu_panel_create",
+    for f in ["oagpu_set_subset", "oagpu_set_covariates", "oagpu_set_interactions", "oagpu_set_dosage_model",
+              "oagpu_load_traits", "oagpu_load_genotypes", "oagpu_scan", "oagpu_last_timings", "oagpu_panel_create",
               "oagpu_server_create", "oagpu_server_submit", "oagpu_server_poll", "oagpu_server_wait"]:
         getattr(L, f).restype = i
     _lib = L

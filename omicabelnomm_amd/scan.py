@@ -52,6 +52,9 @@ class Settings:
     limit_m: int = 2**31 - 1
     limit_t: int = 2**31 - 1
     limit_n: int = 2**31 - 1
+    dosages: bool = False
+    model: int = -1
+    fname_dosages: str = ""
     precision: str = "fp64"
     device: int = 0
 
@@ -80,6 +83,7 @@ class _CSettings(ctypes.Structure):
         ("ini_IDX_interactions", ctypes.c_int), ("end_IDX_interactions", ctypes.c_int),
         ("limit_m", ctypes.c_int), ("limit_t", ctypes.c_int), ("limit_n", ctypes.c_int),
         ("precision", ctypes.c_int), ("device", ctypes.c_int), ("verbose", ctypes.c_int),
+        ("dosages", ctypes.c_int), ("model", ctypes.c_int), ("fname_dosages", ctypes.c_char_p),
     ]
 
 
@@ -145,7 +149,8 @@ class Algorithm:
             ini_IDX_interactions=params.ini_IDX_interactions,
             end_IDX_interactions=params.end_IDX_interactions, limit_m=params.limit_m,
             limit_t=params.limit_t, limit_n=params.limit_n,
-            precision=L.PRECISIONS[params.precision], device=params.device, verbose=int(verbose))
+            precision=L.PRECISIONS[params.precision], device=params.device, verbose=int(verbose),
+            dosages=int(params.dosages), model=params.model, fname_dosages=enc(params.fname_dosages))
         co = _COutputs()
         err = ctypes.create_string_buffer(4096)
         rc = h.omicabelnomm_solve(ctypes.byref(cs), ctypes.byref(co), err, 4096)
@@ -273,6 +278,21 @@ class Scanner:
         assert E.shape[0] == self.n
         self._check(self._L.oagpu_set_interactions(self._ctx, E.ctypes.data, E.shape[0], E.shape[1]))
 
+    def set_dosage_model(self, factors, file_r, sum=True):
+        """Dosage model for the genotype blocks loaded afterwards
+        (src/AIOwrapper.cpp:256-340,1208-1297): each file SNP has file_r
+        columns, the first len(factors) are read; sum=True makes one column
+        per SNP (sum of factor x column), sum=False one column per factor.
+        factors=None switches the model off."""
+        if factors is None:
+            self._check(self._L.oagpu_set_dosage_model(self._ctx, L.DOSAGE_NONE, 1, 1, None))
+            self._dos = None
+            return
+        f = np.ascontiguousarray(factors, dtype=np.float64)
+        mode = L.DOSAGE_SUM if sum else L.DOSAGE_SCALE
+        self._check(self._L.oagpu_set_dosage_model(self._ctx, mode, int(file_r), int(f.size), f.ctypes.data))
+        self._dos = (int(file_r), 1 if sum else int(f.size))
+
     def load_traits(self, Y, trait_offset=0):
         ptr, where, dt, ld, t, keep = _device_ptr(Y)
         self._keep["Y"] = keep
@@ -282,8 +302,12 @@ class Scanner:
     def load_genotypes(self, G, m=None, col_src=None, col_env=None, snp_offset=0):
         ptr, where, dt, ld, n_src, keep = _device_ptr(G)
         self._keep["G"] = keep
+        n_col = n_src
+        dos = getattr(self, "_dos", None)
+        if dos is not None:
+            n_col = (n_src // dos[0]) * dos[1]
         if m is None:
-            m = n_src // self.r if col_src is None else len(col_src) // self.r
+            m = n_col // self.r if col_src is None else len(col_src) // self.r
         src = env = None
         if col_src is not None:
             src = np.ascontiguousarray(col_src, dtype=np.int32)

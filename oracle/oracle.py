@@ -132,3 +132,62 @@ def scan(XL, XR, Y, r, *, min_p_store=0.1, min_p_disp=0.05, min_r2_store=1e-5,
     res["n_emitted"] = out.n_emitted
     res["t_star"] = prm.t_star
     return res
+
+
+# ---------------------------------------------------------------------------
+# Dosage models (host-side in the reference).
+
+def read_dosage_factors(text: str, expected: int):
+    """read_dosages (src/AIOwrapper.cpp:2236-2256): `expected` floats read
+    with operator>> while the stream is not at its end.  Where the reference
+    accepts a list with every factor read, the result is the same.  A read
+    that fails (the end of the file after a trailing blank, or a token that
+    is not a number) leaves the reference's factor uninitialised while still
+    counting it; here such a list is refused.  Returns (ok, factors as
+    float32 values)."""
+    toks = text.split()
+    vals = []
+    for tok in toks[:expected]:
+        try:
+            vals.append(float(np.float32(float(tok))))
+        except ValueError:
+            break
+    return len(vals) == expected, np.array(vals, dtype=np.float64)
+
+
+def dosage_model(model: int, file_r: int, factor_text: str = ""):
+    """Factor vector of a dosage model (src/AIOwrapper.cpp:256-340):
+    model 0 additive (2*AA + AB), 1 dominant (AA + AB), 2 recessive (AA),
+    3 linear from a file (one column per factor), 4 additive from a file.
+    Returns (status, factors, summed): status 0 ok, -1 wrong column count,
+    -2 unreadable factor list.  The reference's recessive model reads an
+    uninitialised second factor (file_r = 3) or no column at all
+    (file_r = 2); here it is the model its own message states,
+    (1*AA, 0*AB, 0*BB)."""
+    if model in (0, 1, 2):
+        if file_r not in (2, 3):
+            return -1, None, True
+        f = {0: [2.0, 1.0], 1: [1.0, 1.0], 2: [1.0]}[model]
+        return 0, np.array(f), True
+    ok, f = read_dosage_factors(factor_text, file_r)
+    if not ok:
+        return -2, None, model == 4
+    return 0, f, model == 4
+
+
+def dosage_apply(G, file_r: int, factors, summed: bool):
+    """Genotype columns of a dosage model (src/AIOwrapper.cpp:1258-1290):
+    G holds file_r columns per SNP; summed: one column per SNP,
+    sum_i f[i] G[:, s*file_r + i]; otherwise f[i] G[:, s*file_r + i] for
+    each factor.  NaN stays NaN (imputed later, as in the reference)."""
+    G = np.asarray(G, dtype=np.float64)
+    n, cols = G.shape
+    assert cols % file_r == 0
+    k = len(factors)
+    blocks = G.reshape(n, cols // file_r, file_r)[:, :, :k]
+    if summed:
+        out = np.zeros((n, cols // file_r))
+        for i in range(k):
+            out = out + factors[i] * blocks[:, :, i]
+        return out
+    return (blocks * np.asarray(factors)[None, None, :]).reshape(n, -1)

@@ -4,7 +4,8 @@ example data (examples/interactions/*.fvi/.fvd, read as DatABEL files).
 Kept: all 4000 rows; XL (4 covariates incl. intercept); the first 100 XR
 columns and the first 50 Y columns -- the sizes the reference's own check
 program scans (tests/test.cpp:199-200: limit_t = 50, limit_m = 100); every
-INT* interaction file.  Run once in a container that has the reference
+INT* interaction file; also copies the dosage factor files
+examples/dosages_{1,2}.txt.  Run once in a container that has the reference
 mounted; the tests only read the .npz.
 """
 import os
@@ -56,6 +57,13 @@ def main():
             out["Enames_" + nm] = np.array(e_names)
     np.savez_compressed(OUT, **out)
     print("wrote", OUT, os.path.getsize(OUT), "bytes")
+    # the check program's dosage factor files (examples/dosages_{1,2}.txt),
+    # data read by its dosage runs, kept byte for byte
+    for k in (1, 2):
+        src = os.path.join(os.path.dirname(SRC), "dosages_%d.txt" % k)
+        dst = os.path.join(os.path.dirname(OUT), "dosages_%d.txt" % k)
+        with open(src, "rb") as fi, open(dst, "wb") as fo:
+            fo.write(fi.read())
 
 
 if __name__ == "__main__":

@@ -22,8 +22,14 @@ def exclusion_keep(n, text):
     return keep
 
 
-def prepare(fx, inter_name=None, multi=False, keep_dep=False, excl_text=None, limit_t=50, limit_m=100):
-    """Returns dict(XL, XR (n x r*m, NaN kept), Y, r, m, E, col_src, col_env, G)."""
+def prepare(fx, inter_name=None, multi=False, keep_dep=False, excl_text=None, limit_t=50, limit_m=100,
+            dosage=None):
+    """Returns dict(XL, XR (n x r*m, NaN kept), Y, r, m, E, col_src, col_env, G, dos).
+
+    dosage: (factors, file_r, summed) -- the genotype file holds file_r
+    columns per SNP and the model's columns (oracle.dosage_apply) take the
+    place of the file columns; G then holds the raw file columns the GPU
+    library combines itself."""
     XL = fx["XL"].astype(np.float64)
     G = fx["XR"][:, :limit_m].astype(np.float64)
     Y = fx["Y"][:, :limit_t].astype(np.float64)
@@ -36,11 +42,21 @@ def prepare(fx, inter_name=None, multi=False, keep_dep=False, excl_text=None, li
         keep &= ~np.isnan(E).any(axis=1)
         E = E[keep]
     XL, G, Y = XL[keep], G[keep], Y[keep]
+    G_raw = G
+    if dosage is not None:
+        from oracle import oracle as O
+        factors, file_r, summed = dosage
+        G = G[:, :(G.shape[1] // file_r) * file_r]
+        G_raw = G
+        G = O.dosage_apply(G, file_r, factors, summed)
+        if not summed and len(factors) > 1:
+            assert E is None, "the reference refuses interactions with more than one column per SNP"
     m_file = G.shape[1]
     if E is None:
-        r, m = 1, m_file
-        col_src = np.arange(m, dtype=np.int32)
-        col_env = -np.ones(m, dtype=np.int32)
+        r = 1 if dosage is None or dosage[2] else len(dosage[0])
+        m = m_file // r
+        col_src = np.arange(m * r, dtype=np.int32)
+        col_env = -np.ones(m * r, dtype=np.int32)
     elif not multi:
         k = E.shape[1]
         r, m = k + int(keep_dep), m_file
@@ -72,8 +88,8 @@ def prepare(fx, inter_name=None, multi=False, keep_dep=False, excl_text=None, li
             v = v * E[:, col_env[j]]
         cols.append(v)
     XR = np.stack(cols, axis=1)
-    return dict(XL=XL, XR=XR, Y=Y, r=r, m=m, E=E, G=G, col_src=col_src, col_env=col_env,
-                n=int(keep.sum()), realN=n)
+    return dict(XL=XL, XR=XR, Y=Y, r=r, m=m, E=E, G=G_raw, col_src=col_src, col_env=col_env,
+                n=int(keep.sum()), realN=n, dos=dosage)
 
 
 # The reference check program's interaction scenarios (tests/test.cpp:201-308),

@@ -124,6 +124,46 @@ def test_dropin_files_basic_model(oracle, fx, tmp_path):
     assert open(os.path.join(tmp, "cli_sto.dbin"), "rb").read() == open(os.path.join(tmp, "basic_sto.dbin"), "rb").read()
 
 
+def test_dropin_files_additive_dosage(oracle, fx, tmp_path):
+    """-j (additive dosage model, 2*AA + AB) with --ngpred 2 through
+    Algorithm.solve and the CLI: the file's 100 columns are 50 SNPs of two
+    probability columns; every output names a SNP by its first column
+    (src/AIOwrapper.cpp:563-588,709-718)."""
+    from omicabelnomm_amd import Algorithm, Outputs
+    tmp = str(tmp_path)
+    _write_inputs(fx, tmp)
+    s = _settings(tmp, "dos")
+    s.r = 2
+    s.dosages = True
+    s.model = 0
+    out = Outputs()
+    Algorithm().solve(s, out)
+    excl = open(os.path.join(GOLDEN, "exclude_individuals.txt")).read()
+    d = prepare(fx, None, excl_text=excl, limit_t=50, limit_m=100, dosage=([2.0, 1.0], 2, True))
+    assert d["m"] == 50
+    o = oracle.scan(d["XL"], d["XR"], d["Y"], 1, real_n=d["realN"], min_p_store=0.00005,
+                    min_p_disp=0.00005, min_r2_store=0.000001, disp_cov=True, threads=4)
+    assert out.total_sig_results == o["n_sig"] and o["n_sig"] > 0
+    rows = read_dis(os.path.join(tmp, "dos_dis.txt"))
+    names = dict(y=list(fx["y_names"]), al=list(fx["xl_names"]), ar=list(fx["xr_names"][::2]))
+    want = expected_dis_rows(o, names, 4, 1, 0.00005, 0.000001, d["realN"])
+    assert len(rows) > 0
+    assert [(r["phe"], r["snp"]) for r in rows] == [(w["phe"], w["snp"]) for w in want]
+    for r, w in zip(rows, want):
+        np.testing.assert_allclose([r["B"], r["SE"], r["T"]], np.float32([w["B"], w["SE"], w["T"]]),
+                                   rtol=2e-6, atol=2e-7)
+    ib = read_ibin(os.path.join(tmp, "dos_sto.ibin"))
+    assert (ib["r"], ib["m"]) == (1, 50)
+    assert ib["ar_names"] == list(fx["xr_names"][::2])
+    cli = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "omicabelnomm_amd", "omicabelnomm")
+    cmd = [cli, "-c", os.path.join(tmp, "XL.fvi"), "-g", os.path.join(tmp, "XR.fvi"), "-p",
+           os.path.join(tmp, "Y.fvi"), "-o", os.path.join(tmp, "cli.txt"), "-x", os.path.join(tmp, "excl.txt"),
+           "-d", "0.00005", "-s", "0.1", "-e", "0.00001", "-r", "0.000001", "-i", "-b", "-n", "2", "-j"]
+    p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout
+    assert open(os.path.join(tmp, "cli_dis.txt")).read() == open(os.path.join(tmp, "dos_dis.txt")).read()
+
+
 @pytest.mark.parametrize("multi", [False, True])
 def test_dropin_files_interactions(oracle, fx, tmp_path, multi):
     from omicabelnomm_amd import Algorithm, Outputs
