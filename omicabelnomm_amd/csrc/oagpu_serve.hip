@@ -84,11 +84,18 @@ struct Request {
 
 }  // namespace
 
+// Streams the per-request kernels of a chunk are spread over (the server's
+// own stream and kLanes - 1 more): a request with few traits launches grids
+// far smaller than the chip, so several requests run side by side.
+constexpr int kLanes = 4;
+
 struct oagpu_server {
   oagpu_panel* panel = nullptr;
   oagpu_server_params prm{};
   std::string err;
   hipStream_t st = nullptr;
+  hipStream_t lane_st[kLanes - 1] = {};
+  hipEvent_t ev_operand = nullptr, ev_lane[kLanes - 1] = {};
   std::thread th;
   std::mutex mu;
   std::condition_variable cv;        // work arrived / stop
@@ -320,9 +327,19 @@ int run_chunk(oagpu_server* S) {
   launch_gemm(prec, S->Ahi, S->Alo, Bhi, Blo, S->Cbuf, mcc, S->ldc, S->n_pad, (int)S->ldop, (int)S->ldop, S->ldc,
               S->st);
 
-  // each request on its slice
+  // each request on its slice; the requests read the shared operand, Xu,
+  // Sbr, column means and contraction output and write only their own
+  // buffers, so they run on several streams, each request on the one with
+  // the fewest traits so far; all of them join the server's stream before
+  // the next chunk overwrites the shared buffers
+  SRVCK(S, hipEventRecord(S->ev_operand, S->st));
+  for (hipStream_t w : S->lane_st) SRVCK(S, hipStreamWaitEvent(w, S->ev_operand, 0));
+  long long lane_load[kLanes] = {0};
   for (Request* R : S->active) {
     oagpu_ctx* c = R->ctx;
+    const int lane = (int)(std::min_element(lane_load, lane_load + kLanes) - lane_load);
+    lane_load[lane] += c->t;
+    hipStream_t st = lane == 0 ? S->st : S->lane_st[lane - 1];
     DevScan d = make_dev(c);
     d.s0 = s0;
     d.mc = mcc;
@@ -342,14 +359,18 @@ int run_chunk(oagpu_server* S) {
     d.snp_offset = 0;
     d.trait_offset = 0;
     d.need_bl = c->prm.disp_cov;
-    if (c->keep) launch_subset_sbr(d, c->Sbr, S->st);
-    launch_snp_gram(d, prec, c->lb, S->st);
-    if (c->compact_corr) launch_corr(d, c->lb, S->st);
-    if (c->use_pst) launch_patsnp(d, c->lb, S->st);
-    launch_reset_counter(c->counters, CNT_FB_CHUNK, S->st);
-    if (c->use_pst) launch_epilogue_pst(d, prec, c->lb, S->st);
-    else launch_epilogue(d, prec, c->lb, S->st);
-    launch_fallback(d, c->lb, S->st);
+    if (c->keep) launch_subset_sbr(d, c->Sbr, st);
+    launch_snp_gram(d, prec, c->lb, st);
+    if (c->compact_corr) launch_corr(d, c->lb, st);
+    if (c->use_pst) launch_patsnp(d, c->lb, st);
+    launch_reset_counter(c->counters, CNT_FB_CHUNK, st);
+    if (c->use_pst) launch_epilogue_pst(d, prec, c->lb, st);
+    else launch_epilogue(d, prec, c->lb, st);
+    launch_fallback(d, c->lb, st);
+  }
+  for (int w = 0; w < kLanes - 1; ++w) {
+    SRVCK(S, hipEventRecord(S->ev_lane[w], S->lane_st[w]));
+    SRVCK(S, hipStreamWaitEvent(S->st, S->ev_lane[w], 0));
   }
   SRVCK(S, hipGetLastError());
   SRVCK(S, hipStreamSynchronize(S->st));
@@ -479,6 +500,9 @@ extern "C" int oagpu_server_create(oagpu_panel* P, const oagpu_server_params* pr
   const size_t pad = (size_t)gemm_row_pad();
   const size_t arows = ((size_t)S->mc + pad - 1) / pad * pad;
   SRVCK(S, hipStreamCreateWithFlags(&S->st, hipStreamNonBlocking));
+  for (hipStream_t& w : S->lane_st) SRVCK(S, hipStreamCreateWithFlags(&w, hipStreamNonBlocking));
+  SRVCK(S, hipEventCreateWithFlags(&S->ev_operand, hipEventDisableTiming));
+  for (hipEvent_t& e : S->ev_lane) SRVCK(S, hipEventCreateWithFlags(&e, hipEventDisableTiming));
   SRVCK(S, hipMalloc(&S->Abuf, arows * S->ldop * S->op_size));
   S->Ahi = S->Abuf;
   S->Alo = prm->precision == OAGPU_BF16X3 ? static_cast<uint16_t*>(S->Abuf) + P->n_pad : nullptr;
@@ -513,6 +537,11 @@ extern "C" void oagpu_server_destroy(oagpu_server* S) {
   (void)hipSetDevice(S->panel->device);
   void* bufs[] = {S->Abuf, S->col_mean, S->Sbr, S->Xu, S->uidx, S->Barena, S->Cbuf};
   for (void* b : bufs) dfree(b);
+  for (hipStream_t w : S->lane_st)
+    if (w) (void)hipStreamDestroy(w);
+  for (hipEvent_t e : S->ev_lane)
+    if (e) (void)hipEventDestroy(e);
+  if (S->ev_operand) (void)hipEventDestroy(S->ev_operand);
   if (S->st) (void)hipStreamDestroy(S->st);
   delete S;
 }
