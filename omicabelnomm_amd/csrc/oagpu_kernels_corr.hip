@@ -19,6 +19,37 @@ constexpr int corr_spl() { return (R == 1 && LB <= 16) ? 2 : 1; }
 template <typename XT, int N> struct XVec { typedef __attribute__((ext_vector_type(N))) XT T; };
 template <typename XT> struct XVec<XT, 1> { typedef XT T; };
 
+// acc += v(lane J of this lane's 16-lane row) * x: one FP64 multiply-add
+// whose first operand is broadcast within each row of 16 lanes by the DPP
+// row_newbcast control (gfx90a and later), so a covariate row held one value
+// per lane feeds every lane without an LDS broadcast per value.  The caller
+// keeps every lane of the wave active, and v comes from an LDS read, not
+// from a VALU write (the DPP read-after-VALU-write hazard).
+#define OA_FMAC_BC(J)                                                                                  \
+  case J:                                                                                              \
+    asm("v_fmac_f64_dpp %0, %1, %2 row_newbcast:" #J " row_mask:0xf bank_mask:0xf"                     \
+        : "+v"(acc) : "v"(v), "v"(x));                                                                 \
+    break;
+template <int J>
+__device__ __forceinline__ void fmac_bcast(double& acc, double v, double x) {
+  static_assert(J >= 0 && J < 16, "row of 16 lanes");
+  switch (J) {
+    OA_FMAC_BC(0) OA_FMAC_BC(1) OA_FMAC_BC(2) OA_FMAC_BC(3) OA_FMAC_BC(4) OA_FMAC_BC(5) OA_FMAC_BC(6)
+    OA_FMAC_BC(7) OA_FMAC_BC(8) OA_FMAC_BC(9) OA_FMAC_BC(10) OA_FMAC_BC(11) OA_FMAC_BC(12) OA_FMAC_BC(13)
+    OA_FMAC_BC(14) OA_FMAC_BC(15)
+  }
+}
+#undef OA_FMAC_BC
+
+template <int LB, int W, int J = 0>
+__device__ __forceinline__ void fmac_row(double (&lr)[LB][W], double v, const double (&xv)[W]) {
+  if constexpr (J < LB) {
+#pragma unroll
+    for (int c = 0; c < W; ++c) fmac_bcast<J>(lr[J][c], v, xv[c]);
+    fmac_row<LB, W, J + 1>(lr, v, xv);
+  }
+}
+
 // generate_correction_missings_Str (src/Algorithm.cpp:1260-1315) for every
 // pattern: C_lr = sum over the pattern's missing rows of xl x', C_rr of x x'.
 // The genotype values come from the compact u-major rows of U written by the
@@ -91,11 +122,18 @@ __global__ __launch_bounds__(256) void k_corr(DevScan d, int gq_count) {
 #pragma unroll
       for (int q = 0; q < CORR_BATCH; ++q) {
         const double* xl = xw[(b0 + q) & (STAGE - 1)];
+        if constexpr (LB <= 16) {
+          // lane j of each 16-lane row holds covariate j (clamped inside
+          // the row); one DPP-broadcast multiply-add per covariate
+          const double v = xl[min(lane & 15, LB - 1)];
+          fmac_row<LB, W>(lr, v, xv[q]);
+        } else {
 #pragma unroll
-        for (int j = 0; j < LB; ++j) {
-          const double a = xl[j];
+          for (int j = 0; j < LB; ++j) {
+            const double a = xl[j];
 #pragma unroll
-          for (int c = 0; c < W; ++c) lr[j][c] += a * xv[q][c];
+            for (int c = 0; c < W; ++c) lr[j][c] += a * xv[q][c];
+          }
         }
 #pragma unroll
         for (int p = 0; p < SPL; ++p)
