@@ -278,7 +278,7 @@ class Scanner:
         assert E.shape[0] == self.n
         self._check(self._L.oagpu_set_interactions(self._ctx, E.ctypes.data, E.shape[0], E.shape[1]))
 
-    def set_dosage_model(self, factors, file_r, sum=True):
+    def set_dosage_model(self, factors, file_r=1, sum=True):
         """Dosage model for the genotype blocks loaded afterwards
         (src/AIOwrapper.cpp:256-340,1208-1297): each file SNP has file_r
         columns, the first len(factors) are read; sum=True makes one column

@@ -140,7 +140,7 @@ def test_dosage_blocks_and_switch_off(oracle):
             compare_fp64(g, o)
         with pytest.raises(OagpuError):
             sc.load_genotypes(np.asfortranarray(G32[:, :74]))
-        sc.set_dosage_model(None, 1)
+        sc.set_dosage_model(None)
         sc.load_genotypes(np.asfortranarray(G32[:, :30]))
         g = sc.scan(dense=True)
         o = oracle.scan(XL, G32[:, :30].astype(np.float64), Y, 1, min_p_store=0.1, min_p_disp=0.05,
