@@ -101,6 +101,14 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError("liboagpu.so is not built (%s); run omicabelnomm_amd.build.build()" % LIB_PATH)
+    # PyTorch ships its own HIP runtime under the same soname as /opt/rocm's.
+    # Whichever is loaded first serves both; when liboagpu.so comes first,
+    # torch's GPU support later finds no device.  Loading torch first (when
+    # it is installed) lets a process use torch tensors beside this library.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     vp, ll, i = ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int
     L.oagpu_create.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(vp)]

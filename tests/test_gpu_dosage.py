@@ -98,6 +98,34 @@ def test_dosage_genotype_types(g_dtype, oracle):
     compare_fp64(g, o)
 
 
+@pytest.mark.parametrize("name", ["additive3", "linear3"])
+def test_dosage_device_input_with_padded_columns(name, oracle):
+    """Genotype probabilities already on the device (float16, leading
+    dimension n + 5): the combine reads the caller's buffer in place."""
+    import torch
+    from omicabelnomm_amd import Scanner
+    dos = MODELS[name]
+    r = 1 if dos[2] else len(dos[0])
+    n = 517
+    XL, G, Y = make_inputs(n=n, l=3, m=23, t=14, file_r=dos[1], seed=89)
+    print("case dosage device", name, flush=True)
+    G16 = G.astype(np.float16)
+    full = torch.zeros((G.shape[1], n + 5), dtype=torch.float16, device="cuda")
+    full[:, :n] = torch.from_numpy(np.ascontiguousarray(G16.T)).to("cuda")
+    Gd = full[:, :n]
+    assert Gd.stride(0) == n + 5
+    with Scanner(n, 3, r, "fp64", min_p_store=0.1, min_p_disp=0.05, min_r2_store=-10, disp_cov=True,
+                 max_traits=64) as sc:
+        sc.set_covariates(XL)
+        sc.set_dosage_model(dos[0], dos[1], sum=dos[2])
+        sc.load_traits(Y)
+        sc.load_genotypes(Gd)
+        g = sc.scan(dense=True)
+    torch.cuda.synchronize()
+    o = oracle_dosage_scan(oracle, XL, G16.astype(np.float64), Y, r, dos)
+    compare_fp64(g, o)
+
+
 # All three probability columns of a SNP sum to 1, the intercept: "linear3"
 # is singular up to rounding, which FP64 parity covers but no precision
 # bound can; the mixed-precision check uses two of the three columns.
