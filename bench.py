@@ -171,7 +171,7 @@ def measured_traffic(variant, config):
 def cpu_baseline(data, budget_s):
     """FP64 CPU restatement (oracle) on the first m' SNPs x all traits."""
     from oracle import oracle as O
-    threads = min(16, os.cpu_count() or 1)   # the GPU box's CPU share
+    threads = min(16, os.cpu_count() or 1)   # the host's CPU share beside one GPU
     XL = data.XL
     Y = data.y_matrix()
 
@@ -326,8 +326,8 @@ def bench_serve(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: every GPU box here has one MI355X and the scan's "
-              "multi-GPU split (SNP-range shards, no collectives) is not part of this build.",
+        print("bench.py runs on one GPU only: the scan's multi-GPU split (SNP-range "
+              "shards, no collectives) is described, not implemented.",
               file=sys.stderr)
         sys.exit(2)
     import torch

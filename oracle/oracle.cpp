@@ -1,8 +1,8 @@
 /*
  * FP64 CPU oracle for the blocked normal-equation OLS scan.
  *
- * TEST INFRASTRUCTURE ONLY: the checker for the HIP path and the CPU
- * restatement timed by bench.py's cpu_baseline leg.  Nothing in the
+ * TEST INFRASTRUCTURE ONLY: what the HIP path's results are compared with,
+ * and the CPU restatement timed by bench.py's cpu_baseline leg.  Nothing in the
  * product path links or loads it.
  *
  * What it restates (all in double precision; the reference computes in

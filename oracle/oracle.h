@@ -3,8 +3,8 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py may load this library, and only as the
- * checker (or as the timed CPU restatement).  The product path never calls
- * it.
+ * reference the HIP results are compared with (or as the timed CPU
+ * restatement).  The product path never calls it.
  *
  * It restates, in double precision, the arithmetic of OmicABELnoMM's
  * partialNEQ_Blocked_STL_MD (src/Algorithm.cpp:322-886) including the

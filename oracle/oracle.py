@@ -1,7 +1,7 @@
 """ctypes wrapper of the FP64 CPU oracle (oracle/oracle.cpp).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
-the checker, and by bench.py's cpu_baseline leg.  The product path
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() to
+compare its scan with, and by bench.py's cpu_baseline leg.  The product path
 (omicabelnomm_amd) never imports this module.
 """
 from __future__ import annotations

@@ -50,7 +50,7 @@ def dpp_hazards(asm_lines):
     return n, bad
 
 
-def test_checker_finds_a_planted_hazard():
+def test_hazard_scan_finds_a_planted_hazard():
     ok = ["ds_read2_b64 v[38:41], v34 offset1:6", "s_waitcnt lgkmcnt(0)",
           "v_fmac_f64_dpp v[2:3], v[38:39], v[20:21] row_newbcast:1 row_mask:0xf bank_mask:0xf"]
     assert dpp_hazards(ok) == (1, [])
