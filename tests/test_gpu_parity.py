@@ -227,3 +227,30 @@ def test_async_block_pipeline_matches_single_scans():
     for a, b in zip(got, want):
         assert len(a) > 0
         assert a.tobytes() == b.tobytes()
+
+
+# Shapes far below one tile: n under and at the 64-row padding, one SNP, one
+# trait, more traits than one wave's lanes; n - missing - p stays >= 3 so the
+# reference's statistics are defined (it requires p < n).
+TINY = {
+    "n9_m1_t1": dict(n=9, l=2, m=1, t=1, miss="none", planted=0),
+    "n50_gxe_m3_t2": dict(n=50, l=3, m=3, t=2, interaction=True, miss="mcar", miss_rate=0.04, planted=0),
+    "n64_l1_m64_t1": dict(n=64, l=1, m=64, t=1, miss="groups", pattern_groups=1, pattern_rate=0.05,
+                          planted=1, beta=0.8),
+    "n65_m5_t65": dict(n=65, l=4, m=5, t=65, miss="mcar", miss_rate=0.03, planted=3, beta=0.8),
+}
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+@pytest.mark.parametrize("name", list(TINY))
+def test_tiny_shapes(name, precision, oracle_mod):
+    data = synth.make(seed=91, **TINY[name])
+    print("case tiny", name, precision, flush=True)
+    g = run_gpu(data, precision, min_r2_store=-10, disp_cov=True)
+    o = run_oracle(oracle_mod, data, min_r2_store=-10, disp_cov=True)
+    if precision == "fp64":
+        compare_fp64(g, o)
+    else:
+        db, dl = precision_report(g, o)
+        assert db < 1e-3 and dl < 1e-3, (db, dl)
+        np.testing.assert_array_equal(g.dense["n_used"], o["n_used"])
