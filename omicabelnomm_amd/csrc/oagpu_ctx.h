@@ -103,6 +103,7 @@ struct oagpu_ctx {
   uint8_t *pd_kept = nullptr, *pd_em = nullptr;
 
   // chunk buffers
+  size_t chunk_budget = 0;   // bytes of chunk buffers a scan may size for (set at the first scan)
   int mc_cap = 0;
   void* Abuf = nullptr;
   void *Ahi = nullptr, *Alo = nullptr, *Cbuf = nullptr;   // Ahi, Alo: views into Abuf
