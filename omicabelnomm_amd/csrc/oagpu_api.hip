@@ -450,13 +450,20 @@ extern "C" int oagpu_load_traits(oagpu_ctx* c, const void* Y, int dtype, int whe
       rep_of[y] = it->second;
     }
   }
-  HIPCK(c, hipMemcpyAsync(c->scratch_i, rep_of.data(), sizeof(int) * t, hipMemcpyHostToDevice, c->st));
-  int* mism_d = c->ypat;   // ypat is rewritten below
-  launch_verify_patterns(t, n, c->rows, c->scratch_i, c->ymiss, mism_d, c->st);
-  HIPCK(c, hipGetLastError());
-  std::vector<int> mism(t);
-  HIPCK(c, hipMemcpyAsync(mism.data(), mism_d, sizeof(int) * t, hipMemcpyDeviceToHost, c->st));
-  HIPCK(c, hipStreamSynchronize(c->st));
+  // Only a trait grouped with another and missing some rows can mismatch
+  // its group's rows; without one (nothing missing, or every pattern
+  // distinct) the device check and its round trip are skipped.
+  bool verify = false;
+  for (int y = 0; y < t && !verify; ++y) verify = rep_of[y] != y && ymiss[y] > 0;
+  std::vector<int> mism(t, 0);
+  if (verify) {
+    HIPCK(c, hipMemcpyAsync(c->scratch_i, rep_of.data(), sizeof(int) * t, hipMemcpyHostToDevice, c->st));
+    int* mism_d = c->ypat;   // ypat is rewritten below
+    launch_verify_patterns(t, n, c->rows, c->scratch_i, c->ymiss, mism_d, c->st);
+    HIPCK(c, hipGetLastError());
+    HIPCK(c, hipMemcpyAsync(mism.data(), mism_d, sizeof(int) * t, hipMemcpyDeviceToHost, c->st));
+    HIPCK(c, hipStreamSynchronize(c->st));
+  }
   std::vector<int> pat_of_rep(t, -1), ypat(t), pat_rep, pat_cnt;
   for (int y = 0; y < t; ++y) {
     const int rep = mism[y] ? y : rep_of[y];
