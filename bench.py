@@ -57,6 +57,9 @@ def parse():
     ap.add_argument("--host-fed", type=int, default=0, metavar="BLOCKS",
                     help="also time the scan with genotypes in page-locked host memory, uploaded in BLOCKS "
                          "blocks, each upload overlapping the previous block's scan (configs 0-2)")
+    ap.add_argument("--dosage", type=int, default=0, metavar="SNPS",
+                    help="also time the device-side dosage combine (additive model, float16 probability "
+                         "columns P(AA), P(AB), P(BB) of SNPS SNPs at the workload's n)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 1 GiB device copy first (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     return ap.parse_args()
@@ -96,6 +99,32 @@ def timed(sc, data, steps, warmup):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
     return dt, res, gemm_ms / max(chunks, 1), chunks / steps
+
+
+def dosage_leg(n, snps, reps=5):
+    """The dosage combine alone: float16 probability columns (3 per SNP)
+    already on the device, additive model 2*P(AA) + P(AB), one FP64 column
+    per SNP.  Algorithmic bytes per element of the output: two float16
+    reads and one FP64 write (12 B)."""
+    import torch
+    from omicabelnomm_amd import Scanner
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(0x0D05)
+    P = torch.rand((snps * 3, n), generator=gen, device="cuda", dtype=torch.float32).to(torch.float16)
+    XL = np.column_stack([np.ones(n), np.random.default_rng(5).standard_normal((n, 2))])
+    with Scanner(n, 3, 1, "bf16x3", max_traits=8) as sc:
+        sc.set_covariates(XL)
+        sc.set_dosage_model([2.0, 1.0], 3, sum=True)
+        sc.load_genotypes(P)          # allocates the combined block
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            sc.load_genotypes(P)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+    nbytes = 12.0 * n * snps
+    return {"snps": snps, "n": n, "ms_per_block": dt * 1e3, "algorithmic_gb": nbytes / 1e9,
+            "gb_per_s": nbytes / dt / 1e9, "frac_of_hbm_peak": nbytes / dt / 8e12}
 
 
 def host_fed(data, precision, p_store, blocks, steps):
@@ -391,6 +420,7 @@ def main():
                 "max_abs_dlog10p": dl, "max_dbeta_over_se": db}
 
     hostfed = host_fed(data, args.precision, args.p_store, args.host_fed, args.steps) if args.host_fed else None
+    dosage = dosage_leg(data.n, args.dosage) if args.dosage else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -428,6 +458,8 @@ def main():
     }
     if hostfed:
         line["host_fed"] = hostfed
+    if dosage:
+        line["dosage_combine"] = dosage
     print(json.dumps(line))
 
 

@@ -330,46 +330,66 @@ void launch_panel_rows(const void* G, int gdt, long long ldg, int s0, int mc, in
 
 // Dosage model columns, formed once per genotype block before the scan
 // (src/AIOwrapper.cpp:1258-1290: an sgemm of the n_used probability
-// columns with the factor vector, or each column times its factor).  One
-// thread per output element, individuals fastest, so reads and writes are
-// contiguous runs; HBM-bound, n_used reads and one FP64 write per element.
+// columns with the factor vector, or each column times its factor).  Output
+// columns on the grid's y axis (one division per column), individuals on x
+// with four consecutive rows per thread: one vector load per source column
+// and one 32-byte store when the block is aligned (vec), element by element
+// otherwise.  HBM-bound, n_used reads and one FP64 write per element.
 template <typename GT>
-__global__ __launch_bounds__(256) void k_dosage(const GT* __restrict__ G, long long ldg, int n, int n_snp,
-                                                DosageModel dm, double* __restrict__ out, long long ldo) {
+__global__ __launch_bounds__(256) void k_dosage(const GT* __restrict__ G, long long ldg, int n, long long ncol,
+                                                DosageModel dm, double* __restrict__ out, long long ldo, int vec) {
+  typedef typename GVec<GT>::T GV;
   const int per = dm.sum ? 1 : dm.n_used;
-  const long long total = (long long)n_snp * per * n;
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
-    const long long col = e / n;
-    const int k = (int)(e - col * n);
+  const int k0 = (blockIdx.x * 256 + threadIdx.x) * 4;
+  if (k0 >= n) return;   // no barrier follows
+  const bool full = vec && k0 + 4 <= n;
+  for (long long col = blockIdx.y; col < ncol; col += gridDim.y) {
     const long long s = col / per;
     const int j = (int)(col - s * per);
-    const GT* g = G + (size_t)(s * dm.file_r) * ldg + k;
-    double v;
-    if (dm.sum) {
-      v = 0.0;
-      for (int i = 0; i < dm.n_used; ++i) v += dm.f[i] * (double)g[(size_t)i * ldg];
-    } else {
-      v = dm.f[j] * (double)g[(size_t)j * ldg];
+    const GT* g = G + (size_t)(s * dm.file_r) * ldg + k0;
+    double* o = out + (size_t)col * ldo + k0;
+    const int i0 = dm.sum ? 0 : j, i1 = dm.sum ? dm.n_used : j + 1;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = i0; i < i1; ++i) {
+      const GT* gi = g + (size_t)i * ldg;
+      if (full) {
+        const GV x = *reinterpret_cast<const GV*>(gi);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v[e] += dm.f[i] * (double)x[e];
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (k0 + e < n) v[e] += dm.f[i] * (double)gi[e];
+      }
     }
-    out[(size_t)col * ldo + k] = v;
+    if (full) {
+      *reinterpret_cast<d4*>(o) = d4{v[0], v[1], v[2], v[3]};
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (k0 + e < n) o[e] = v[e];
+    }
   }
 }
 
 void launch_dosage(const void* G, int gdt, long long ldg, int n, int n_snp, const DosageModel& dm, double* out,
                    long long ldo, hipStream_t st) {
-  const long long total = (long long)n_snp * (dm.sum ? 1 : dm.n_used) * n;
-  if (total <= 0) return;
-  const int blocks = (int)std::min<long long>((total + 255) / 256, 1 << 16);
+  const long long ncol = (long long)n_snp * (dm.sum ? 1 : dm.n_used);
+  if (ncol <= 0 || n <= 0) return;
+  const size_t esz = gdt == DT_F64 ? 8 : (gdt == DT_F32 ? 4 : 2);
+  // vector path: 4-row runs of every column start on a 4-element boundary
+  // and the output rows (ldo, a multiple of 4) on a 32-byte one
+  const int vec = ((uintptr_t)G % (4 * esz) == 0) && (ldg % 4 == 0) && ((uintptr_t)out % 32 == 0) && (ldo % 4 == 0);
+  const dim3 grid((unsigned)((n + 1023) / 1024), (unsigned)std::min<long long>(ncol, 65535));
   if (gdt == DT_F64)
-    hipLaunchKernelGGL(k_dosage<double>, dim3(blocks), dim3(256), 0, st, static_cast<const double*>(G), ldg, n,
-                       n_snp, dm, out, ldo);
+    hipLaunchKernelGGL(k_dosage<double>, grid, dim3(256), 0, st, static_cast<const double*>(G), ldg, n, ncol, dm, out,
+                       ldo, vec);
   else if (gdt == DT_F32)
-    hipLaunchKernelGGL(k_dosage<float>, dim3(blocks), dim3(256), 0, st, static_cast<const float*>(G), ldg, n,
-                       n_snp, dm, out, ldo);
+    hipLaunchKernelGGL(k_dosage<float>, grid, dim3(256), 0, st, static_cast<const float*>(G), ldg, n, ncol, dm, out,
+                       ldo, vec);
   else
-    hipLaunchKernelGGL(k_dosage<_Float16>, dim3(blocks), dim3(256), 0, st, static_cast<const _Float16*>(G), ldg,
-                       n, n_snp, dm, out, ldo);
+    hipLaunchKernelGGL(k_dosage<_Float16>, grid, dim3(256), 0, st, static_cast<const _Float16*>(G), ldg, n, ncol, dm,
+                       out, ldo, vec);
 }
 
 void launch_prep(const DevScan& d, int prec, void* Ahi, void* Alo, bool corr, hipStream_t st) {
