@@ -52,16 +52,17 @@ class ScanData:
         materialised, NaN kept (what the reference's I/O thread hands to the
         scan, src/AIOwrapper.cpp:767-840)."""
         snps = np.arange(self.m) if snps is None else np.asarray(snps)
-        G = self.G.detach().cpu().double().numpy()
+        js = (snps[:, None] * self.r + np.arange(self.r)[None, :]).reshape(-1)
+        srcs = np.asarray(self.col_src)[js] if self.col_src is not None else js
+        # only the source columns used, gathered where G lives
+        uniq, inv = np.unique(srcs, return_inverse=True)
+        G = self.G[torch.as_tensor(uniq, device=self.G.device)].detach().cpu().double().numpy()
         cols = []
-        for s in snps:
-            for c in range(self.r):
-                j = s * self.r + c
-                src = self.col_src[j] if self.col_src is not None else j
-                v = G[src].copy()
-                if self.col_env is not None and self.col_env[j] >= 0:
-                    v = v * self.E[:, self.col_env[j]]
-                cols.append(v)
+        for q, j in enumerate(js):
+            v = G[inv[q]].copy()
+            if self.col_env is not None and self.col_env[j] >= 0:
+                v = v * self.E[:, self.col_env[j]]
+            cols.append(v)
         return np.stack(cols, axis=1) if cols else np.zeros((self.n, 0))
 
     def y_matrix(self, traits=None) -> np.ndarray:

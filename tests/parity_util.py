@@ -44,6 +44,38 @@ def mlog10(p):
         return -np.log10(p)
 
 
+def mlog10p_t(t):
+    """-log10 of the p-value rule of hpc_statistics (src/Algorithm.cpp:
+    1159-1167) computed from |t| without forming p: 0.5 - 0.1 (4.4 - t) t
+    below t = 1.6, erfc(t / sqrt 2) = 2 Phi(-t) above, whose logarithm
+    log 2 + log Phi(-t) stays exact where p itself is a subnormal double
+    (|t| > ~37.5) or underflows."""
+    from scipy.special import log_ndtr
+    t = np.abs(np.asarray(t, dtype=np.float64))
+    lo = t < 1.6
+    with np.errstate(divide="ignore", invalid="ignore"):
+        small = -np.log10(0.5 - 0.1 * (4.4 - t) * t)
+    big = -(np.log(2.0) + log_ndtr(-t)) / LOG10
+    return np.where(lo, small, big)
+
+
+T_SWITCH = 1.6
+
+
+def dlog10p_t(ta, tb):
+    """|d -log10 p| between two t statistics under the reference's p-value
+    rule, and the pairs it leaves out.  The rule is discontinuous at
+    t = 1.6 (0.5 - 0.1 (4.4 - t) t = 0.052 just below, erfc(1.6 / sqrt 2)
+    = 0.110 at 1.6), so two t values 1e-6 apart on either side of it differ
+    by 0.32 in -log10 p; such straddling pairs are measured by |dt| instead.
+    Returns (dl over the other pairs, |dt| over the straddling pairs)."""
+    ta = np.abs(np.asarray(ta, dtype=np.float64))
+    tb = np.abs(np.asarray(tb, dtype=np.float64))
+    straddle = (ta < T_SWITCH) != (tb < T_SWITCH)
+    dl = np.abs(mlog10p_t(ta[~straddle]) - mlog10p_t(tb[~straddle]))
+    return dl, np.abs(ta[straddle] - tb[straddle])
+
+
 def dlog10p(pa, pb):
     """|-log10 pa + log10 pb|, 0 where both underflowed to 0."""
     d = np.abs(mlog10(pa) - mlog10(pb))
@@ -67,8 +99,17 @@ def compare_fp64(g, o, *, rtol=1e-10):
     assert err_b.max(initial=0) <= rtol, ("beta", err_b.max())
     assert err_se.max(initial=0) <= rtol, ("se", err_se.max())
     kept = o["kept"].astype(bool)
-    dl = dlog10p(d["pval"][kept], o["pval"][kept])
+    # p as stored, where it is a normal double; -log10 p from t everywhere
+    normal = kept & (o["pval"] > 1e-300)
+    dl = dlog10p(d["pval"][normal], o["pval"][normal])
     assert dl.max(initial=0) <= 1e-8, ("-log10 p", dl.max())
+    # -log10 p ~ t^2 / (2 ln 10): a relative error e in t moves it by about
+    # 2 e (-log10 p), which is large for the near-perfect fits' t ~ 1e7
+    tk = d["tstat"][kept]
+    dlt, dt_sw = dlog10p_t(tk, o["tstat"][kept])
+    lim = 1e-8 + 4.0 * rtol * mlog10p_t(o["tstat"][kept])[(np.abs(tk) < T_SWITCH) == (np.abs(o["tstat"][kept]) < T_SWITCH)]
+    assert np.all(dlt <= lim), ("-log10 p from t", dlt.max())
+    assert dt_sw.max(initial=0) <= 1e-9, ("t at the p-rule switch", dt_sw.max())
     r2ok = ~np.isnan(o["r2"])
     np.testing.assert_allclose(d["r2"][r2ok], o["r2"][r2ok], rtol=1e-9, atol=1e-12)
     assert g.stats["n_sig"] == o["n_sig"]
@@ -92,8 +133,7 @@ def precision_report(g, o):
     d = g.dense
     ok = ~np.isnan(o["se"]) & ~np.isnan(d["se"])
     db = np.abs(d["beta"][ok] - o["beta"][ok]) / o["se"][ok]
-    from oracle.oracle import pvalue
-    pv_o = np.vectorize(pvalue)(o["tstat"][ok])
-    pv_g = np.vectorize(pvalue)(d["tstat"][ok])
-    dl = dlog10p(pv_g, pv_o)
+    dl, dt_sw = dlog10p_t(d["tstat"][ok], o["tstat"][ok])
+    # t straddling the reference's p-rule switch: held to |dt| instead
+    assert dt_sw.max(initial=0) <= 1e-3, ("t at the p-rule switch", dt_sw.max())
     return float(db.max(initial=0)), float(dl.max(initial=0))

@@ -118,6 +118,7 @@ def test_mixed_precision_bounds(precision, oracle_mod):
     o = run_oracle(oracle_mod, data, min_r2_store=-10)
     db, dl = precision_report(g, o)
     print(precision, "max|db|/SE", db, "max|d-log10p|", dl, flush=True)
+    assert db <= 1e-3 and dl <= 1e-3, (db, dl)
     assert db < 1e-3
     assert dl < 1e-3
     # hit set at p < 1e-8 identical

@@ -185,3 +185,17 @@ def test_synth_generator_shapes():
     miss = np.isnan(Y)
     # 2 groups, each with its own row pattern
     assert (miss[:, 0] == miss[:, 1]).all() and not (miss[:, 0] == miss[:, 4]).all()
+
+
+def test_mlog10p_from_t_matches_the_pvalue_rule(oracle):
+    """The tests' -log10 p taken from t (no subnormal p) agrees with the
+    oracle's p-value rule wherever that p is a normal double, on both sides
+    of the t = 1.6 switch."""
+    from parity_util import mlog10p_t
+    ts = np.concatenate([np.linspace(0.0, 1.59, 40), np.linspace(1.6, 37.0, 200)])
+    want = np.array([-np.log10(oracle.pvalue(t)) for t in ts])
+    np.testing.assert_allclose(mlog10p_t(ts), want, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(mlog10p_t(-ts), want, rtol=1e-12, atol=1e-12)
+    # beyond the normal range it keeps growing smoothly (p would underflow)
+    big = mlog10p_t(np.array([40.0, 60.0]))
+    assert np.all(np.isfinite(big)) and big[1] > big[0] > 340
