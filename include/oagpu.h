@@ -185,6 +185,24 @@ int oagpu_scan_async(oagpu_ctx *ctx, int want_dense);
 int oagpu_scan_wait(oagpu_ctx *ctx, oagpu_dense *dense, oagpu_hit *hits,
                     long long hit_cap, oagpu_stats *stats);
 
+/* oagpu_scan without device allocation.  No counterpart in the reference
+ * (which scans on the CPU): it is for callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc.  It runs the same kernels as oagpu_scan on the
+ * context's loaded covariates, traits and genotypes (set up beforehand) and
+ * returns the same hits and statistics, but its chunk buffers (operand rows,
+ * contraction output, Gram pieces, corrections, counters, fallback list) live
+ * in one static device array of OAGPU_NOALLOC_SCRATCH_BYTES per GPU, held by
+ * the library until the process ends; SNP chunks are sized to fit it.  The
+ * contraction is the precision's default hand-written kernel (no autotuner,
+ * no library GEMM).  A workload whose single-SNP chunk does not fit returns
+ * OAGPU_ERR_ARG with the sizes in oagpu_last_error.  No dense outputs.  All
+ * calls on one GPU share its array: calls that overlap on one GPU (other
+ * host threads, other contexts) race. */
+#define OAGPU_NOALLOC_SCRATCH_BYTES (128ull << 20)
+int oagpu_scan_noalloc(oagpu_ctx *ctx, oagpu_hit *hits, long long hit_cap,
+                       oagpu_stats *stats);
+
 /* Page-locked host buffers for genotype and trait blocks (hipHostMalloc):
  * uploads from them run at full PCIe rate and asynchronously.  A host block
  * passed to oagpu_load_genotypes may be refilled once the scan that reads

@@ -161,6 +161,7 @@ extern "C" int oagpu_create(const oagpu_params* prm, oagpu_ctx** out) {
   if (prm->precision < 0 || prm->precision > 2) return fail(c, OAGPU_ERR_ARG, "bad precision");
   if (prm->device < 0 || prm->device >= ndev) return fail(c, OAGPU_ERR_ARG, "bad device");
   HIPCK(c, hipSetDevice(prm->device));
+  noalloc_setup(prm->device);
   HIPCK(c, hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
   HIPCK(c, hipStreamCreateWithFlags(&c->cst, hipStreamNonBlocking));
   for (int i = 0; i < 2; ++i) {
@@ -707,6 +708,44 @@ static int ensure_chunk(oagpu_ctx* c, int mc) {
   return OAGPU_OK;
 }
 
+int oa::enqueue_chunks(oagpu_ctx* c, DevScan& d, long long mc, void* Ahi, void* Alo, hipEvent_t* E,
+                       bool fixed_gemm) {
+  const int r = c->r, m = c->m;
+  launch_reset_counters(d.counters, CNT_N, c->st);
+  const int nchunks = (int)((m + mc - 1) / mc);
+  for (int ci = 0; ci < nchunks; ++ci) {
+    const int s0 = (int)(ci * mc);
+    const int mcc = (int)std::min<long long>(mc, m - s0);
+    d.s0 = s0;
+    d.mc = mcc;
+    hipEvent_t* Ec = E ? E + (size_t)ci * 7 : nullptr;
+    if (Ec) HIPCK(c, hipEventRecord(Ec[0], c->st));
+    launch_prep(d, c->prm.precision, Ahi, Alo, c->compact_corr, c->st);
+    if (Ec) HIPCK(c, hipEventRecord(Ec[1], c->st));
+    void* C = const_cast<void*>(d.Cbuf);
+    if (fixed_gemm)
+      launch_gemm_fixed(c->prm.precision, Ahi, Alo, c->Bhi, c->Blo, C, mcc * r, out_cols(c), c->n_pad,
+                        (int)c->ldop, (int)c->ldop, out_cols(c), c->st);
+    else
+      launch_gemm(c->prm.precision, Ahi, Alo, c->Bhi, c->Blo, C, mcc * r, out_cols(c), c->n_pad, (int)c->ldop,
+                  (int)c->ldop, out_cols(c), c->st);
+    if (Ec) HIPCK(c, hipEventRecord(Ec[2], c->st));
+    launch_snp_gram(d, c->prm.precision, c->lb, c->st);
+    if (Ec) HIPCK(c, hipEventRecord(Ec[3], c->st));
+    if (c->compact_corr) launch_corr(d, c->lb, c->st);
+    if (c->use_pst) launch_patsnp(d, c->lb, c->st);
+    if (Ec) HIPCK(c, hipEventRecord(Ec[4], c->st));
+    launch_reset_counter(d.counters, CNT_FB_CHUNK, c->st);
+    if (c->use_pst) launch_epilogue_pst(d, c->prm.precision, c->lb, c->st);
+    else launch_epilogue(d, c->prm.precision, c->lb, c->st);
+    if (Ec) HIPCK(c, hipEventRecord(Ec[5], c->st));
+    launch_fallback(d, c->lb, c->st);
+    if (Ec) HIPCK(c, hipEventRecord(Ec[6], c->st));
+    HIPCK(c, hipGetLastError());
+  }
+  return OAGPU_OK;
+}
+
 static int scan_enqueue(oagpu_ctx* c, bool want_dense) {
   if (c->pending) return fail(c, OAGPU_ERR_STATE, "a scan is already in flight (oagpu_scan_wait first)");
   if (!c->have_traits || !c->have_geno) return fail(c, OAGPU_ERR_STATE, "load traits and genotypes before scanning");
@@ -758,8 +797,6 @@ static int scan_enqueue(oagpu_ctx* c, bool want_dense) {
     HIPCK(c, hipStreamWaitEvent(c->st, c->up_ev[c->gslot], 0));
     c->upload_pending = false;
   }
-  launch_reset_counters(c->counters, CNT_N, c->st);
-
   const int nchunks = (int)((m + mc - 1) / mc);
   // timing events, kept across scans
   while (c->pev.size() < (size_t)nchunks * 7) {
@@ -771,32 +808,8 @@ static int scan_enqueue(oagpu_ctx* c, bool want_dense) {
   d.need_bl = c->prm.disp_cov || want_dense;
   d.d_beta = c->pd_beta; d.d_se = c->pd_se; d.d_t = c->pd_t; d.d_p = c->pd_p; d.d_kept = c->pd_kept;
   d.d_r2 = c->pd_r2; d.d_rss = c->pd_rss; d.d_emitted = c->pd_em; d.d_varx = c->pd_varx;
-  for (int ci = 0; ci < nchunks; ++ci) {
-    const int s0 = (int)(ci * mc);
-    const int mcc = (int)std::min<long long>(mc, m - s0);
-    d.s0 = s0;
-    d.mc = mcc;
-    hipEvent_t* E = &c->pev[(size_t)ci * 7];
-    HIPCK(c, hipEventRecord(E[0], c->st));
-    launch_prep(d, c->prm.precision, c->Ahi, c->Alo, c->compact_corr, c->st);
-    HIPCK(c, hipEventRecord(E[1], c->st));
-    launch_gemm(c->prm.precision, c->Ahi, c->Alo, c->Bhi, c->Blo, c->Cbuf, mcc * r, out_cols(c), c->n_pad,
-                (int)c->ldop, (int)c->ldop, out_cols(c),
-                c->st);
-    HIPCK(c, hipEventRecord(E[2], c->st));
-    launch_snp_gram(d, c->prm.precision, c->lb, c->st);
-    HIPCK(c, hipEventRecord(E[3], c->st));
-    if (c->compact_corr) launch_corr(d, c->lb, c->st);
-    if (c->use_pst) launch_patsnp(d, c->lb, c->st);
-    HIPCK(c, hipEventRecord(E[4], c->st));
-    launch_reset_counter(c->counters, CNT_FB_CHUNK, c->st);
-    if (c->use_pst) launch_epilogue_pst(d, c->prm.precision, c->lb, c->st);
-    else launch_epilogue(d, c->prm.precision, c->lb, c->st);
-    HIPCK(c, hipEventRecord(E[5], c->st));
-    launch_fallback(d, c->lb, c->st);
-    HIPCK(c, hipEventRecord(E[6], c->st));
-    HIPCK(c, hipGetLastError());
-  }
+  rc = enqueue_chunks(c, d, mc, c->Ahi, c->Alo, c->pev.data(), false);
+  if (rc) return rc;
   c->gemm_variant = gemm_variant_name(c->prm.precision, (int)std::min<long long>(mc, m) * r, out_cols(c), c->n_pad);
   // the genotype slot may be overwritten once everything above has run
   HIPCK(c, hipEventRecord(c->free_ev[c->gslot], c->st));
@@ -804,6 +817,7 @@ static int scan_enqueue(oagpu_ctx* c, bool want_dense) {
   HIPCK(c, hipMemcpyAsync(c->cnt_h, c->counters, sizeof(unsigned long long) * CNT_N, hipMemcpyDeviceToHost, c->st));
   HIPCK(c, hipEventRecord(c->done_ev, c->st));
   c->pending = true;
+  c->p_timed = true;
   c->p_dense = want_dense;
   c->p_nchunks = nchunks;
   c->p_t = t;
@@ -811,14 +825,14 @@ static int scan_enqueue(oagpu_ctx* c, bool want_dense) {
   return OAGPU_OK;
 }
 
-static int scan_collect(oagpu_ctx* c, oagpu_dense* dense, oagpu_hit* hits, long long hit_cap,
-                        oagpu_stats* stats) {
+int oa::scan_collect(oagpu_ctx* c, oagpu_dense* dense, oagpu_hit* hits, long long hit_cap,
+                     oagpu_stats* stats) {
   if (!c->pending) return fail(c, OAGPU_ERR_STATE, "no scan in flight");
   c->pending = false;
   HIPCK(c, hipEventSynchronize(c->done_ev));
   const int p = c->p, t = c->p_t, m = c->p_m, nchunks = c->p_nchunks;
   for (int i = 0; i < 8; ++i) c->timings[i] = 0.0;
-  for (int ci = 0; ci < nchunks; ++ci) {
+  for (int ci = 0; ci < (c->p_timed ? nchunks : 0); ++ci) {
     hipEvent_t* E = &c->pev[(size_t)ci * 7];
     for (int k = 0; k < 6; ++k) {
       float ms = 0.f;
@@ -826,7 +840,7 @@ static int scan_collect(oagpu_ctx* c, oagpu_dense* dense, oagpu_hit* hits, long 
       c->timings[k] += ms;
     }
   }
-  {
+  if (c->p_timed) {
     float ms = 0.f;
     HIPCK(c, hipEventElapsedTime(&ms, c->pev[0], c->pev[(size_t)(nchunks - 1) * 7 + 6]));
     c->timings[6] = ms;

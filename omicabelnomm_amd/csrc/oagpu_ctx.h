@@ -93,6 +93,7 @@ struct oagpu_ctx {
 
   // a scan in flight (oagpu_scan_async .. oagpu_scan_wait)
   bool pending = false;
+  bool p_timed = true;   // per-chunk timing events recorded
   bool p_dense = false;
   int p_nchunks = 0, p_t = 0, p_m = 0;
   std::vector<hipEvent_t> pev;
@@ -169,6 +170,19 @@ int ensure_pair_buffers(oagpu_ctx* c, int mc);
 // contraction; FP32 below the FP32 and split-BF16 contractions, whose own
 // rounding (2^-17 and coarser) is far above the FP32 rounding of an imputed
 // genotype (2^-24 relative), while it halves the corrections' gather traffic.
+// Kernels of every chunk of the loaded block on c->st, mc SNPs per chunk:
+// counters reset, then per chunk the operand pass, contraction, Gram
+// pieces, corrections, pair solve and statistics, fallback.  d carries the
+// chunk buffers, Ahi/Alo the operand rows; E: 7 timing events per chunk, or
+// nullptr; fixed_gemm: the default hand-written contraction kernel, launched
+// without the autotuner (no timing launches, no library handle).
+int enqueue_chunks(oagpu_ctx* c, DevScan& d, long long mc, void* Ahi, void* Alo, hipEvent_t* E, bool fixed_gemm);
+// Waits for the enqueued scan and returns its hits (sorted), statistics and
+// dense outputs; the scan's status.
+int scan_collect(oagpu_ctx* c, oagpu_dense* dense, oagpu_hit* hits, long long hit_cap, oagpu_stats* stats);
+// Looks up this device's no-allocation scratch array once (oagpu_noalloc.hip);
+// called by oagpu_create with the device current.
+void noalloc_setup(int device);
 inline int xu_dtype(const oagpu_ctx* c) { return c->prm.precision == OAGPU_FP64 ? DT_F64 : DT_F32; }
 inline size_t xu_size(const oagpu_ctx* c) { return xu_dtype(c) == DT_F64 ? sizeof(double) : sizeof(float); }
 }  // namespace oa

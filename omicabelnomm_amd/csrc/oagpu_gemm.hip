@@ -642,6 +642,14 @@ const char* gemm_variant_name(int prec, int M, int N, int K) {
 }
 
 int gemm_row_pad() { return 256; }
+int gemm_fixed_row_pad() { return 128; }
+
+void launch_gemm_fixed(int prec, const void* Ahi, const void* Alo, const void* Bhi, const void* Blo, void* C, int M,
+                       int N, int K, int lda, int ldb, int ldc, hipStream_t st) {
+  if (prec == PREC_FP64) v_f64(Ahi, Alo, Bhi, Blo, C, M, N, K, lda, ldb, ldc, st);
+  else if (prec == PREC_FP32) v_f32(Ahi, Alo, Bhi, Blo, C, M, N, K, lda, ldb, ldc, st);
+  else v_bf_lds(Ahi, Alo, Bhi, Blo, C, M, N, K, lda, ldb, ldc, st);
+}
 
 void launch_gemm(int prec, const void* Ahi, const void* Alo, const void* Bhi, const void* Blo,
                  void* C, int M, int N, int K, int lda, int ldb, int ldc, hipStream_t st) {

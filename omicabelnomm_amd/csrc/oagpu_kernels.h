@@ -49,6 +49,9 @@ void launch_epilogue_pst(const DevScan& d, int prec, int lb, hipStream_t st);
 int patsnp_width(int r, int l, int lb);
 void launch_fallback(const DevScan& d, int lb, hipStream_t st);
 void launch_reset_counter(unsigned long long* c, int idx, hipStream_t st);
+// zero bytes (a multiple of 4) at p with a kernel (no memset node when the
+// caller captures the stream)
+void launch_zero(void* p, size_t bytes, hipStream_t st);
 // counters [0, n), n <= 64, in one launch
 void launch_reset_counters(unsigned long long* c, int n, hipStream_t st);
 // r = 1: out[s] = sum over the subset's rows (d.keep) of x^2
@@ -61,7 +64,13 @@ void launch_subset_sbr(const DevScan& d, double* out, hipStream_t st);
 // [lo | hi], strides 2K.  The variant is chosen by timing.
 void launch_gemm(int prec, const void* Ahi, const void* Alo, const void* Bhi, const void* Blo,
                  void* C, int M, int N, int K, int lda, int ldb, int ldc, hipStream_t st);
+// The default hand-written kernel of the precision (FP64 and FP32 direct,
+// split-BF16 LDS-staged 128 x 128), launched without the autotuner.
+void launch_gemm_fixed(int prec, const void* Ahi, const void* Alo, const void* Bhi, const void* Blo,
+                       void* C, int M, int N, int K, int lda, int ldb, int ldc, hipStream_t st);
 const char* gemm_variant_name(int prec, int M, int N, int K);
 int gemm_row_pad();
+// A rows read by launch_gemm_fixed: M rounded up to its 128-row tiles
+int gemm_fixed_row_pad();
 
 }  // namespace oa

@@ -12,6 +12,8 @@
 //   oagpu_kernels_epi.hip   hpc_statistics (src/Algorithm.cpp:714-792,
 //                           1034-1214, 1370-1399)
 // Everything here is FP64.
+#include <algorithm>
+
 #include "oagpu_kernels.h"
 #include "oagpu_geno.h"
 #include "oagpu_pair.h"
@@ -192,6 +194,10 @@ __global__ __launch_bounds__(256) void k_fallback(DevScan d) {
 }
 
 __global__ void k_reset_counter(unsigned long long* c, int idx) { c[idx] = 0ull; }
+// words 32-bit zeros from p (4-byte aligned), grid-stride
+__global__ __launch_bounds__(256) void k_zero_words(uint32_t* p, size_t words) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < words; i += (size_t)gridDim.x * 256) p[i] = 0u;
+}
 __global__ void k_reset_counters(unsigned long long* c, int n) {
   if ((int)threadIdx.x < n) c[threadIdx.x] = 0ull;
 }
@@ -244,6 +250,13 @@ void launch_subset_sbr(const DevScan& d, double* out, hipStream_t st) {
 
 void launch_reset_counters(unsigned long long* c, int n, hipStream_t st) {
   hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, c, n);
+}
+
+void launch_zero(void* p, size_t bytes, hipStream_t st) {
+  const size_t words = bytes / 4;
+  if (words == 0) return;
+  const unsigned blocks = (unsigned)std::min<size_t>((words + 255) / 256, 1024);
+  hipLaunchKernelGGL(k_zero_words, dim3(blocks), dim3(256), 0, st, static_cast<uint32_t*>(p), words);
 }
 
 void launch_reset_counter(unsigned long long* c, int idx, hipStream_t st) {

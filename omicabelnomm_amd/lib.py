@@ -89,7 +89,8 @@ EXPORTS = ["oagpu_create", "oagpu_destroy", "oagpu_last_error", "oagpu_set_subse
            "oagpu_scan", "oagpu_scan_async", "oagpu_scan_wait", "oagpu_host_alloc", "oagpu_host_free",
            "oagpu_last_timings", "oagpu_gemm_variant", "oagpu_tvalue",
            "oagpu_panel_create", "oagpu_panel_destroy", "oagpu_server_create", "oagpu_server_destroy",
-           "oagpu_server_last_error", "oagpu_server_submit", "oagpu_server_poll", "oagpu_server_wait"]
+           "oagpu_server_last_error", "oagpu_server_submit", "oagpu_server_poll", "oagpu_server_wait",
+           "oagpu_scan_noalloc"]
 
 _lib = None
 
@@ -124,6 +125,7 @@ def load():
     L.oagpu_load_traits.argtypes = [vp, vp, i, i, ll, i, i]
     L.oagpu_load_genotypes.argtypes = [vp, vp, i, i, ll, i, i, vp, vp, i]
     L.oagpu_scan.argtypes = [vp, ctypes.POINTER(Dense), vp, ll, ctypes.POINTER(Stats)]
+    L.oagpu_scan_noalloc.argtypes = [vp, vp, ll, ctypes.POINTER(Stats)]
     L.oagpu_scan_async.argtypes = [vp, i]
     L.oagpu_scan_wait.argtypes = [vp, ctypes.POINTER(Dense), vp, ll, ctypes.POINTER(Stats)]
     L.oagpu_last_timings.argtypes = [vp, ctypes.POINTER(ctypes.c_double), i]
@@ -144,7 +146,7 @@ def load():
     L.oagpu_server_wait.argtypes = [vp, ll, vp, ll, ctypes.POINTER(Stats), ctypes.POINTER(ctypes.c_double),
                                     ctypes.c_char_p, i]
     for f in ["oagpu_set_subset", "oagpu_set_covariates", "oagpu_set_interactions", "oagpu_set_dosage_model",
-              "oagpu_load_traits", "oagpu_load_genotypes", "oagpu_scan", "oagpu_last_timings", "oagpu_panel_create",
+              "oagpu_load_traits", "oagpu_load_genotypes", "oagpu_scan", "oagpu_scan_noalloc", "oagpu_last_timings", "oagpu_panel_create",
               "oagpu_server_create", "oagpu_server_submit", "oagpu_server_poll", "oagpu_server_wait"]:
         getattr(L, f).restype = i
     _lib = L
