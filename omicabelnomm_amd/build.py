@@ -22,7 +22,7 @@ CLI = os.path.join(HERE, "omicabelnomm")
 
 HIP_SOURCES = ["oagpu_api.hip", "oagpu_kernels_trait.hip", "oagpu_kernels_prep.hip", "oagpu_kernels_snp.hip",
                "oagpu_kernels_corr.hip", "oagpu_kernels_pst.hip", "oagpu_kernels_epi.hip", "oagpu_gemm.hip",
-               "oagpu_gemm_lib.hip", "oagpu_serve.hip", "oagpu_noalloc.hip"]
+               "oagpu_gemm_lib.hip", "oagpu_gemm_fused.hip", "oagpu_serve.hip", "oagpu_noalloc.hip"]
 # library GEMM candidates of the contraction autotuner
 HIP_LIBS = ["-L/opt/rocm/lib", "-lhipblaslt", "-Wl,-rpath,/opt/rocm/lib"]
 HOST_SOURCES = ["host/databel.cpp", "host/writers.cpp", "host/driver.cpp"]

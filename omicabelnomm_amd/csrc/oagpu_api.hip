@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -179,6 +180,11 @@ extern "C" int oagpu_create(const oagpu_params* prm, oagpu_ctx** out) {
   c->ldop = (long long)c->n_pad * (prm->precision == OAGPU_BF16X3 ? 2 : 1);
   c->hit_cap = prm->max_hits > 0 ? prm->max_hits : (1ll << 22);
   c->fb_cap = 1 << 16;
+  {
+    std::vector<float> ones((size_t)c->n_pad, 1.0f);
+    HIPCK(c, hipMalloc(&c->E32, sizeof(float) * ones.size()));
+    HIPCK(c, hipMemcpy(c->E32, ones.data(), sizeof(float) * ones.size(), hipMemcpyHostToDevice));
+  }
   HIPCK(c, hipMalloc(&c->hits, sizeof(oagpu_hit) * (size_t)c->hit_cap));
   HIPCK(c, hipMalloc(&c->counters, sizeof(unsigned long long) * CNT_N));
   HIPCK(c, hipMalloc(&c->fb, sizeof(FallbackEntry) * (size_t)c->fb_cap));
@@ -227,7 +233,7 @@ extern "C" void oagpu_destroy(oagpu_ctx* c) {
   void* dense_bufs[] = {c->pd_beta, c->pd_se, c->pd_t, c->pd_p, c->pd_kept, c->pd_r2, c->pd_rss, c->pd_em,
                         c->pd_varx};
   for (void* b : dense_bufs) dfree(b);
-  void* bufs[] = {c->XL_cm, c->XL_rm, c->U0, c->Stl_inv_d, c->Stl_pk, c->XLop_hi, c->XLop_lo, c->E,
+  void* bufs[] = {c->XL_cm, c->XL_rm, c->U0, c->Stl_inv_d, c->Stl_pk, c->XLop_hi, c->XLop_lo, c->E, c->E32,
                   c->Y_own, c->Bbuf, c->rows, c->ymiss, c->yzero, c->ypat, c->yy, c->sst,
                   c->rss0, c->rt, c->zt, c->ay, c->hash, c->scratch_i, c->pat_cnt, c->pat_rep, c->pat_pd, c->keep,
                   c->UA, c->UA_invd, c->Abuf, c->Cbuf,
@@ -388,6 +394,13 @@ extern "C" int oagpu_set_interactions(oagpu_ctx* c, const double* E, long long l
     }
   HIPCK(c, hipMalloc(&c->E, sizeof(double) * e.size()));
   HIPCK(c, hipMemcpy(c->E, e.data(), sizeof(double) * e.size(), hipMemcpyHostToDevice));
+  std::vector<float> e32((size_t)(n_env + 1) * np, 0.0f);
+  for (size_t q = 0; q < e.size(); ++q) e32[q] = (float)e[q];
+  for (size_t k = 0; k < np; ++k) e32[(size_t)n_env * np + k] = 1.0f;
+  dfree(c->E32);
+  c->E32 = nullptr;
+  HIPCK(c, hipMalloc(&c->E32, sizeof(float) * e32.size()));
+  HIPCK(c, hipMemcpy(c->E32, e32.data(), sizeof(float) * e32.size(), hipMemcpyHostToDevice));
   c->lde = (long long)np;
   c->n_env = n_env;
   return OAGPU_OK;
@@ -634,6 +647,13 @@ extern "C" int oagpu_load_genotypes(oagpu_ctx* c, const void* G, int dtype, int 
     // identity descriptors: kernels read col_src == nullptr as s*r + c
     c->col_src = c->col_env = nullptr;
   }
+  c->pair_env = -1;
+  if (r == 2 && col_src && col_env && col_env[0] >= 0) {
+    bool pair = true;
+    for (long long s = 0; s < m && pair; ++s)
+      pair = col_src[2 * s] == col_src[2 * s + 1] && col_env[2 * s] == col_env[0] && col_env[2 * s + 1] < 0;
+    if (pair) c->pair_env = col_env[0];
+  }
   HIPCK(c, hipEventRecord(c->up_ev[slot], c->cst));
   c->gslot = slot;
   c->upload_pending = true;
@@ -708,6 +728,43 @@ static int ensure_chunk(oagpu_ctx* c, int mc) {
   return OAGPU_OK;
 }
 
+// Contraction path per shape class: the fused split-BF16 kernel (the
+// operand pass then writes no operand) or the operand pass plus the
+// autotuned contraction.  Chosen by timing both once on the real operands;
+// OAGPU_GEMM_VARIANT pins one (the fused kernel's name, or any other).
+namespace {
+struct PathKey {
+  int mclass, nclass, k, gdt, r;
+  bool operator<(const PathKey& o) const {
+    return std::tie(mclass, nclass, k, gdt, r) < std::tie(o.mclass, o.nclass, o.k, o.gdt, o.r);
+  }
+};
+std::map<PathKey, bool> g_path;
+std::mutex g_path_mu;
+PathKey path_key(const oagpu_ctx* c, int M, int N) {
+  int nc = 64;
+  while (nc < N) nc *= 2;
+  return PathKey{M >= 65536 ? 2 : (M >= 4096 ? 1 : 0), nc, c->n_pad, c->gdt, c->r};
+}
+}  // namespace
+
+static bool fused_capable(const oagpu_ctx* c) {
+  return c->prm.precision == OAGPU_BF16X3 && !c->keep && c->pair_env >= 0 && c->col_src &&
+         gemm_fused_supported(c->gdt, c->Gm, c->ldg, c->n, c->n_pad);
+}
+
+static FusedGemmArgs fused_args(const oagpu_ctx* c, const DevScan& d, int M) {
+  FusedGemmArgs a{};
+  a.G = d.Gm; a.ldg = d.ldg; a.n = c->n; a.K = c->n_pad;
+  a.col_src = d.col_src; a.s0 = d.s0;
+  a.E = c->E32 + (size_t)c->pair_env * c->n_pad;
+  a.col_mean = d.col_mean;
+  a.B = static_cast<const uint16_t*>(c->Blo); a.ldb = c->ldop;
+  a.C = static_cast<float*>(const_cast<void*>(d.Cbuf)); a.ldc = out_cols(c);
+  a.M = M; a.N = out_cols(c);
+  return a;
+}
+
 int oa::enqueue_chunks(oagpu_ctx* c, DevScan& d, long long mc, void* Ahi, void* Alo, hipEvent_t* E,
                        bool fixed_gemm) {
   const int r = c->r, m = c->m;
@@ -719,16 +776,74 @@ int oa::enqueue_chunks(oagpu_ctx* c, DevScan& d, long long mc, void* Ahi, void* 
     d.s0 = s0;
     d.mc = mcc;
     hipEvent_t* Ec = E ? E + (size_t)ci * 7 : nullptr;
-    if (Ec) HIPCK(c, hipEventRecord(Ec[0], c->st));
-    launch_prep(d, c->prm.precision, Ahi, Alo, c->compact_corr, c->st);
-    if (Ec) HIPCK(c, hipEventRecord(Ec[1], c->st));
     void* C = const_cast<void*>(d.Cbuf);
-    if (fixed_gemm)
-      launch_gemm_fixed(c->prm.precision, Ahi, Alo, c->Bhi, c->Blo, C, mcc * r, out_cols(c), c->n_pad,
-                        (int)c->ldop, (int)c->ldop, out_cols(c), c->st);
-    else
-      launch_gemm(c->prm.precision, Ahi, Alo, c->Bhi, c->Blo, C, mcc * r, out_cols(c), c->n_pad, (int)c->ldop,
-                  (int)c->ldop, out_cols(c), c->st);
+    const int M = mcc * r, N = out_cols(c);
+    auto run_lib = [&]() {
+      launch_prep(d, c->prm.precision, Ahi, Alo, c->compact_corr, c->st);
+      launch_gemm(c->prm.precision, Ahi, Alo, c->Bhi, c->Blo, C, M, N, c->n_pad, (int)c->ldop, (int)c->ldop, N,
+                  c->st);
+    };
+    auto run_fused = [&]() {
+      launch_prep(d, PREC_SIDE, nullptr, nullptr, c->compact_corr, c->st);
+      launch_gemm_fused(fused_args(c, d, M), c->gdt, c->st);
+    };
+    bool fused = false;
+    if (!fixed_gemm && fused_capable(c)) {
+      const char* pin = getenv("OAGPU_GEMM_VARIANT");
+      if (pin) {
+        fused = !strcmp(pin, kFusedVariant);
+      } else {
+        const PathKey key = path_key(c, M, N);
+        int known = -1;
+        {
+          std::lock_guard<std::mutex> lk(g_path_mu);
+          auto it = g_path.find(key);
+          if (it != g_path.end()) known = it->second ? 1 : 0;
+        }
+        if (known < 0) {
+          // time both paths once (each after a warm-up run, which also lets
+          // the contraction autotuner settle), keep the faster
+          hipEvent_t t0, t1, t2, t3;
+          HIPCK(c, hipEventCreate(&t0));
+          HIPCK(c, hipEventCreate(&t1));
+          HIPCK(c, hipEventCreate(&t2));
+          HIPCK(c, hipEventCreate(&t3));
+          run_lib();
+          HIPCK(c, hipEventRecord(t0, c->st));
+          run_lib();
+          HIPCK(c, hipEventRecord(t1, c->st));
+          run_fused();
+          HIPCK(c, hipEventRecord(t2, c->st));
+          run_fused();
+          HIPCK(c, hipEventRecord(t3, c->st));
+          HIPCK(c, hipEventSynchronize(t3));
+          float ms_lib = 0.f, ms_fused = 0.f;
+          HIPCK(c, hipEventElapsedTime(&ms_lib, t0, t1));
+          HIPCK(c, hipEventElapsedTime(&ms_fused, t2, t3));
+          for (hipEvent_t e : {t0, t1, t2, t3}) (void)hipEventDestroy(e);
+          known = ms_fused <= ms_lib ? 1 : 0;
+          std::lock_guard<std::mutex> lk(g_path_mu);
+          g_path[key] = known == 1;
+        }
+        fused = known == 1;
+      }
+    }
+    c->last_fused = fused;
+    if (Ec) HIPCK(c, hipEventRecord(Ec[0], c->st));
+    if (fused) {
+      launch_prep(d, PREC_SIDE, nullptr, nullptr, c->compact_corr, c->st);
+      if (Ec) HIPCK(c, hipEventRecord(Ec[1], c->st));
+      launch_gemm_fused(fused_args(c, d, M), c->gdt, c->st);
+    } else {
+      launch_prep(d, c->prm.precision, Ahi, Alo, c->compact_corr, c->st);
+      if (Ec) HIPCK(c, hipEventRecord(Ec[1], c->st));
+      if (fixed_gemm)
+        launch_gemm_fixed(c->prm.precision, Ahi, Alo, c->Bhi, c->Blo, C, M, N, c->n_pad, (int)c->ldop,
+                          (int)c->ldop, N, c->st);
+      else
+        launch_gemm(c->prm.precision, Ahi, Alo, c->Bhi, c->Blo, C, M, N, c->n_pad, (int)c->ldop, (int)c->ldop, N,
+                    c->st);
+    }
     if (Ec) HIPCK(c, hipEventRecord(Ec[2], c->st));
     launch_snp_gram(d, c->prm.precision, c->lb, c->st);
     if (Ec) HIPCK(c, hipEventRecord(Ec[3], c->st));
@@ -810,7 +925,9 @@ static int scan_enqueue(oagpu_ctx* c, bool want_dense) {
   d.d_r2 = c->pd_r2; d.d_rss = c->pd_rss; d.d_emitted = c->pd_em; d.d_varx = c->pd_varx;
   rc = enqueue_chunks(c, d, mc, c->Ahi, c->Alo, c->pev.data(), false);
   if (rc) return rc;
-  c->gemm_variant = gemm_variant_name(c->prm.precision, (int)std::min<long long>(mc, m) * r, out_cols(c), c->n_pad);
+  c->gemm_variant = c->last_fused ? kFusedVariant
+                                  : gemm_variant_name(c->prm.precision, (int)std::min<long long>(mc, m) * r,
+                                                      out_cols(c), c->n_pad);
   // the genotype slot may be overwritten once everything above has run
   HIPCK(c, hipEventRecord(c->free_ev[c->gslot], c->st));
   c->free_valid[c->gslot] = true;

@@ -38,6 +38,9 @@ struct oagpu_ctx {
   double* E = nullptr;
   long long lde = 0;
   int n_env = 0;
+  // FP32 copy for the fused contraction: n_env rows, then a row of ones (the
+  // factor of a column without interaction), each n_pad long (zero past n)
+  float* E32 = nullptr;
 
   // traits
   bool have_traits = false;
@@ -85,6 +88,9 @@ struct oagpu_ctx {
   int gslot = 1;
   bool upload_pending = false;
   int *col_src = nullptr, *col_env = nullptr;
+  // the block's SNPs are [E*G, G] pairs of one genotype column, all with
+  // interaction column pair_env (-1: another layout)
+  int pair_env = -1;
   // dosage model: the combined FP64 columns of each slot's block
   bool dos_on = false;
   oa::DosageModel dos{};
@@ -124,6 +130,7 @@ struct oagpu_ctx {
   int fb_cap = 0;
   double timings[8] = {0};
   std::string gemm_variant = "none";
+  bool last_fused = false;   // the last scan's contraction read the genotypes itself
 };
 
 #define HIPCK(ctx, expr)                                                              \

@@ -635,10 +635,23 @@ static int n_class(int N) {
   return c;
 }
 
+// The variant launch_gemm runs for this shape: the pinned one, the timed
+// winner, or the only candidate of the precision.
 const char* gemm_variant_name(int prec, int M, int N, int K) {
-  std::lock_guard<std::mutex> lk(g_tune_mu);
-  auto it = g_tuned.find(TuneKey{prec, m_class(M), n_class(N), K});
-  return it == g_tuned.end() ? "untuned" : kVariants[it->second].name;
+  const int pv = pinned_variant(prec);
+  if (pv >= 0) return kVariants[pv].name;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(TuneKey{prec, m_class(M), n_class(N), K});
+    if (it != g_tuned.end()) return kVariants[it->second].name;
+  }
+  int nvar = 0, only = -1;
+  for (int c = 0; c < kNumVariants; ++c)
+    if (kVariants[c].prec == prec && kVariants[c].tuned) {
+      ++nvar;
+      only = c;
+    }
+  return nvar == 1 ? kVariants[only].name : "untuned";
 }
 
 int gemm_row_pad() { return 256; }

@@ -68,6 +68,27 @@ void launch_gemm(int prec, const void* Ahi, const void* Alo, const void* Bhi, co
 // split-BF16 LDS-staged 128 x 128), launched without the autotuner.
 void launch_gemm_fixed(int prec, const void* Ahi, const void* Alo, const void* Bhi, const void* Blo,
                        void* C, int M, int N, int K, int lda, int ldb, int ldc, hipStream_t st);
+// Split-BF16 contraction that reads the genotype columns itself
+// (oagpu_gemm_fused.hip), for blocks whose SNPs are [E*G, G] pairs from one
+// genotype column and one interaction column E: C row 2s = (E*G_s)' B', row
+// 2s+1 = G_s' B', a missing genotype replaced by col_mean of the row.  B rows
+// interleaved [lo | hi] (stride ldb = 2K).  Only for blocks that
+// gemm_fused_supported accepts; M is even (two rows per SNP).
+struct FusedGemmArgs {
+  const void* G; long long ldg; int n, K;
+  const int* col_src; int s0;        // block descriptors; SNP s reads column col_src[2 (s0 + s)]
+  const float* E;                    // the interaction column, K floats, zero past n
+  const double* col_mean;            // [M] chunk-local
+  const uint16_t* B; long long ldb;
+  float* C; long long ldc;
+  int M, N, tiles_n, total;
+};
+void launch_gemm_fused(const FusedGemmArgs& a, int gdt, hipStream_t st);
+// genotypes of element type gdt at G (column stride ldg, n rows, operand
+// depth K) the fused contraction takes: float32 or float16, 4-element runs
+// aligned, n % 4 == 0
+bool gemm_fused_supported(int gdt, const void* G, long long ldg, int n, int K);
+constexpr const char* kFusedVariant = "bf16x3_fused_256x256";
 const char* gemm_variant_name(int prec, int M, int N, int K);
 int gemm_row_pad();
 // A rows read by launch_gemm_fixed: M rounded up to its 128-row tiles

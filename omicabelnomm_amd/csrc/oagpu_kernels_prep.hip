@@ -18,7 +18,9 @@ constexpr int prep_spw() { return R == 1 ? 4 : (R == 2 ? 2 : 1); }
 
 template <int OP>
 __device__ __forceinline__ void store_op4(void* Ahi, void* Alo, size_t o, const double* v) {
-  if (OP == PREC_FP64) {
+  if constexpr (OP == PREC_SIDE) {
+    return;
+  } else if (OP == PREC_FP64) {
     d4 x = {v[0], v[1], v[2], v[3]};
     *reinterpret_cast<d4*>(reinterpret_cast<double*>(Ahi) + o) = x;
   } else if (OP == PREC_FP32) {
@@ -252,6 +254,7 @@ static void prep_rxg(const DevScan& d, int prec, void* Ahi, void* Alo, hipStream
   dim3 grid((nb + 7) / 8 * 8), block(256);   // a multiple of 8 for the XCD mapping
   if (prec == PREC_FP64) hipLaunchKernelGGL((k_prep<R, PREC_FP64, XT, GT>), grid, block, 0, st, d, Ahi, Alo);
   else if (prec == PREC_FP32) hipLaunchKernelGGL((k_prep<R, PREC_FP32, XT, GT>), grid, block, 0, st, d, Ahi, Alo);
+  else if (prec == PREC_SIDE) hipLaunchKernelGGL((k_prep<R, PREC_SIDE, XT, GT>), grid, block, 0, st, d, Ahi, Alo);
   else hipLaunchKernelGGL((k_prep<R, PREC_BF16X3, XT, GT>), grid, block, 0, st, d, Ahi, Alo);
 }
 
@@ -261,6 +264,7 @@ static void prep_rxgo(const DevScan& d, int prec, void* Ahi, void* Alo, hipStrea
   const int nb = (d.mc + prep_spw<R>() - 1) / prep_spw<R>();
   dim3 grid((nb + 7) / 8 * 8), block(256);
   if (prec == PREC_FP32) hipLaunchKernelGGL((k_prep<R, PREC_FP32, XT, GT>), grid, block, 0, st, d, Ahi, Alo);
+  else if (prec == PREC_SIDE) hipLaunchKernelGGL((k_prep<R, PREC_SIDE, XT, GT>), grid, block, 0, st, d, Ahi, Alo);
   else hipLaunchKernelGGL((k_prep<R, PREC_BF16X3, XT, GT>), grid, block, 0, st, d, Ahi, Alo);
 }
 

@@ -133,7 +133,7 @@ GEMM_VARIANTS = {
     "fp64": ["f64_direct_128x128"],
     "fp32": ["f32_direct_128x128", "f32_hipblaslt"],
     "bf16x3": ["bf16x3_hipblaslt", "bf16x3_lds_pf2_128x128", "bf16x3_lds_256x128", "bf16x3_lds_128x128",
-               "bf16x3_direct_128x128"],
+               "bf16x3_direct_128x128", "bf16x3_fused_256x256"],
 }
 
 
@@ -152,6 +152,74 @@ def test_gemm_variant(precision, variant, oracle_mod, monkeypatch):
         db, dl = precision_report(g, o)
         print(variant, "max|db|/SE", db, "max|d-log10p|", dl, flush=True)
         assert db < 1e-3 and dl < 1e-3
+
+
+FUSED_CASES = {
+    # genotype element type, leading-dimension padding (3 breaks the aligned
+    # 4-element runs), missing genotypes, generator shape, chunk size, and
+    # whether the fused kernel takes the block ([E*G, G] pairs, float32 or
+    # float16, aligned runs, n % 4 == 0)
+    "f32_gxe": ("float32", 0, 0.0, dict(interaction=True), 0, True),
+    "f32_gxe_nan_chunks": ("float32", 0, 0.03, dict(interaction=True), 37, True),
+    "f16_gxe_nan": ("float16", 0, 0.02, dict(interaction=True), 0, True),
+    "f16_gxe_pad": ("float16", 3, 0.0, dict(interaction=True), 0, False),
+    "f64_gxe": ("float64", 0, 0.02, dict(interaction=True), 0, False),
+    "f32_gxe_n_odd": ("float32", 0, 0.0, dict(interaction=True, n=1111), 0, False),
+    "f32_r1": ("float32", 0, 0.02, dict(r=1), 0, False),
+}
+
+
+@pytest.mark.parametrize("name", list(FUSED_CASES))
+def test_fused_contraction(name, oracle_mod, monkeypatch):
+    """The split-BF16 contraction that reads the genotype columns itself
+    ([E*G, G] pairs): float32 and float16 genotypes, missing genotypes (the
+    column means substituted in registers), n not a multiple of the k-step,
+    two tiles of SNPs (the second ragged) and two of traits, ragged chunks --
+    within the mixed-precision bounds of the oracle, identical p < 1e-8
+    hits, and within 1e-5 SE of the LDS kernel fed by the operand pass.
+    Blocks it does not take (other element types, unaligned columns,
+    n % 4 != 0, no interaction) run the operand pass and the pinned kernel
+    falls back to the autotuned one."""
+    import torch
+    g_dtype, ld_pad, gnan, shape, max_snps, taken = FUSED_CASES[name]
+    kw = dict(n=1100, l=4, m=200, t=300, miss="groups", pattern_groups=3, pattern_rate=0.03, planted=20,
+              beta=0.3, geno_nan=gnan, seed=61, g_dtype=getattr(torch, g_dtype))
+    kw.update(shape)
+    data = synth.make(**kw)
+    print("case fused", name, flush=True)
+    n = data.n
+    full = torch.zeros((data.G.shape[0], n + ld_pad), dtype=data.G.dtype, device="cuda")
+    full[:, :n] = data.G.to("cuda")
+    data.G = full[:, :n]
+    res = {}
+    for v in ("bf16x3_fused_256x256", "bf16x3_lds_128x128"):
+        monkeypatch.setenv("OAGPU_GEMM_VARIANT", v)
+        from omicabelnomm_amd import Scanner
+        with Scanner(n, data.l, data.r, "bf16x3", min_r2_store=-10, max_traits=data.t, max_snps=max_snps) as sc:
+            sc.set_covariates(data.XL)
+            if data.E is not None:
+                sc.set_interactions(data.E)
+            sc.load_traits(data.Y.to("cuda").contiguous())
+            sc.load_genotypes(data.G, m=data.m, col_src=data.col_src, col_env=data.col_env)
+            res[v] = sc.scan(dense=True)
+            if v == "bf16x3_lds_128x128" or taken:
+                assert sc.gemm_variant() == v
+            else:
+                assert sc.gemm_variant() != v
+    o = run_oracle(oracle_mod, data, min_r2_store=-10)
+    g = res["bf16x3_fused_256x256"]
+    db, dl = precision_report(g, o)
+    print(name, "max|db|/SE", db, "max|d-log10p|", dl, flush=True)
+    assert db < 1e-3 and dl < 1e-3
+    d, e = g.dense, res["bf16x3_lds_128x128"].dense
+    ok = ~np.isnan(o["se"])
+    np.testing.assert_array_equal(ok, ~np.isnan(d["se"]))
+    # the fused kernel against the LDS kernel: the same products in another
+    # order; a fallback runs the autotuned kernel, held to the 1e-4 used
+    # between contraction kernels elsewhere
+    assert np.max(np.abs(d["beta"][ok] - e["beta"][ok]) / e["se"][ok]) < (1e-5 if taken else 1e-4)
+    hit = lambda x: (x["pval"] < 1e-8) & x["kept"].astype(bool)
+    np.testing.assert_array_equal(hit(d), hit(o))
 
 
 @pytest.mark.parametrize("g_dtype", ["float16", "float32", "float64"])
