@@ -129,6 +129,7 @@ DevScan oa::make_dev(const oagpu_ctx* c) {
   d.ypat = c->ypat; d.rt = c->rt; d.zt = c->zt; d.ay = c->ay;
   d.pat_cnt = c->pat_cnt; d.pat_rep = c->pat_rep; d.rows = c->rows;
   d.UA = c->UA; d.UA_invd = c->UA_invd; d.pat_pd = c->pat_pd;
+  d.wtrait = c->wtrait; d.wpat = c->wpat; d.n_waves = c->n_waves;
   d.nU = c->compact_corr ? c->nU : 0;
   d.uidx = c->uidx; d.XLu = c->XLu; d.pu_off = c->pu_off; d.pu_list = c->pu_list;
   d.Xu = c->Xu; d.xu_dt = xu_dtype(c); d.ldxu = c->ldxu; d.pst = c->pst;
@@ -236,7 +237,7 @@ extern "C" void oagpu_destroy(oagpu_ctx* c) {
   void* bufs[] = {c->XL_cm, c->XL_rm, c->U0, c->Stl_inv_d, c->Stl_pk, c->XLop_hi, c->XLop_lo, c->E, c->E32,
                   c->Y_own, c->Bbuf, c->rows, c->ymiss, c->yzero, c->ypat, c->yy, c->sst,
                   c->rss0, c->rt, c->zt, c->ay, c->hash, c->scratch_i, c->pat_cnt, c->pat_rep, c->pat_pd, c->keep,
-                  c->UA, c->UA_invd, c->Abuf, c->Cbuf,
+                  c->UA, c->UA_invd, c->wtrait, c->wpat, c->Abuf, c->Cbuf,
                   c->col_mean, c->Sbr, c->Str, c->varx, c->Ccorr, c->hits, c->counters, c->fb,
                   c->uidx, c->pu_off, c->pu_list, c->XLu, c->Xu, c->pst};
   for (void* b : bufs) dfree(b);
@@ -504,6 +505,39 @@ extern "C" int oagpu_load_traits(oagpu_ctx* c, const void* Y, int dtype, int whe
   HIPCK(c, hipMemcpyAsync(c->pat_cnt, pat_cnt.data(), sizeof(int) * G, hipMemcpyHostToDevice, c->st));
   HIPCK(c, hipMemcpyAsync(c->pat_rep, pat_rep.data(), sizeof(int) * G, hipMemcpyHostToDevice, c->st));
   HIPCK(c, hipMemcpyAsync(c->ypat, ypat.data(), sizeof(int) * t, hipMemcpyHostToDevice, c->st));
+  {
+    // Trait waves of the pattern-table pair kernel.  A pattern's traits fill
+    // waves of 64 of their own, and its remainder too when it holds 32 or
+    // more traits, so such a wave reads its (SNP, pattern) rows with scalar
+    // loads; the smaller remainders share mixed waves, which read per lane.
+    std::vector<std::vector<int>> by(G);
+    for (int y = 0; y < t; ++y) by[ypat[y]].push_back(y);
+    std::vector<int> wtrait, wpat, pool;
+    auto emit = [&](const int* ys, size_t k, int g) {
+      wpat.push_back(g);
+      for (size_t i = 0; i < 64; ++i) wtrait.push_back(i < k ? ys[i] : -1);
+    };
+    for (int g = 0; g < G; ++g) {
+      const std::vector<int>& v = by[g];
+      size_t i = 0;
+      for (; i + 64 <= v.size(); i += 64) emit(v.data() + i, 64, g);
+      if (v.size() - i >= 32) emit(v.data() + i, v.size() - i, g);
+      else pool.insert(pool.end(), v.begin() + i, v.end());
+    }
+    for (size_t i = 0; i < pool.size(); i += 64) emit(pool.data() + i, std::min<size_t>(64, pool.size() - i), -1);
+    c->n_waves = (int)wpat.size();
+    if (wpat.size() > c->wave_cap) {
+      HIPCK(c, hipStreamSynchronize(c->st));
+      dfree(c->wtrait); dfree(c->wpat);
+      c->wave_cap = wpat.size();
+      HIPCK(c, hipMalloc(&c->wtrait, sizeof(int) * 64 * c->wave_cap));
+      HIPCK(c, hipMalloc(&c->wpat, sizeof(int) * c->wave_cap));
+    }
+    HIPCK(c, hipMemcpyAsync(c->wtrait, wtrait.data(), sizeof(int) * wtrait.size(), hipMemcpyHostToDevice, c->st));
+    HIPCK(c, hipMemcpyAsync(c->wpat, wpat.data(), sizeof(int) * wpat.size(), hipMemcpyHostToDevice, c->st));
+    // the host vectors go out of scope: finish the copies first
+    HIPCK(c, hipStreamSynchronize(c->st));
+  }
   d = make_dev(c);
   launch_pattern(d, c->Stl_pk, G, c->pat_rep, c->UA, c->UA_invd, c->pat_pd, c->st);
   HIPCK(c, hipGetLastError());

@@ -68,6 +68,11 @@ struct oagpu_ctx {
   int *pat_cnt = nullptr, *pat_rep = nullptr, *pat_pd = nullptr;
   double *UA = nullptr, *UA_invd = nullptr;
   size_t pat_cap = 0;
+  // trait waves of the pattern-table pair kernel: 64 trait ids per wave (-1
+  // for an idle lane) and the wave's pattern (-1: several patterns)
+  int *wtrait = nullptr, *wpat = nullptr;
+  int n_waves = 0;
+  size_t wave_cap = 0;
 
   // genotypes: host blocks are uploaded on a second stream into one of two
   // device slots, so the next block's copy overlaps the current scan

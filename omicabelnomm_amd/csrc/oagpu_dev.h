@@ -67,6 +67,11 @@ struct DevScan {
   const double* UA;      // [l(l+1)/2][G] packed chol(Stl - C_ll)
   const double* UA_invd; // [l][G] 1/diag
   const int* pat_pd;     // [G] 1 if Stl - C_ll is positive definite
+  // trait waves: the traits of a pattern with many traits fill waves of their
+  // own; the others share mixed waves
+  const int* wtrait;     // [n_waves][64] trait id, -1 for an idle lane
+  const int* wpat;       // [n_waves] the wave's pattern, -1 if mixed
+  int n_waves;
 
   // compact missing rows: U = union of the patterns' missing rows (sorted)
   int nU;                // |U| (0: corrections read the genotypes directly)

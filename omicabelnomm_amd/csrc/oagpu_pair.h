@@ -47,14 +47,15 @@ __device__ __forceinline__ TraitK trait_k(const DevScan& d, int y) {
 
 // Coefficient vectors use a split layout: b[0..LB) covariates (index h < l
 // valid), b[LB..LB+R) the SNP terms (coefficient index h = l + c), so every
-// register index is a compile-time constant.
-template <int R, int LB>
-__device__ void stats_emit(const DevScan& d, const TraitK& tk, int y, int s, const double* b, double rss) {
+// register index is a compile-time constant.  vx points at the SNP's var_x
+// row (p values); a caller whose SNP is wave-uniform passes it in the
+// constant address space so the values come through scalar loads.
+template <int R, int LB, typename VX>
+__device__ void stats_emit(const DevScan& d, const TraitK& tk, int y, int s, const double* b, double rss, VX vx) {
   const int l = d.l, p = d.p;
   const int ymiss = tk.ymiss;
   const double vf = tk.vf;
   const double R2 = 1.0 - rss * tk.r2s;
-  const double* vx = d.varx + (size_t)s * p;
   const size_t pair = (size_t)y * d.m_total + (size_t)(d.s0 + s);
   const bool dense = d.d_beta != nullptr;
   const double qnan = __longlong_as_double(0x7ff8000000000000ll);
@@ -77,10 +78,9 @@ __device__ void stats_emit(const DevScan& d, const TraitK& tk, int y, int s, con
   unsigned long long mask = 0;
   int nk = 0, nsig = 0;
   bool has_snp = false;
-  double sig = 0.0;
+  double sg2 = 0.0;   // sigma^2; its square root only for what is kept
   if (rss > 0.0 && (R2 > d.min_r2_store || d.store_all)) {
-    const double sg2 = rss * tk.inv_nc;
-    sig = sqrt(sg2);
+    sg2 = rss * tk.inv_nc;
     const double s2 = sg2 * vf;
     const double tt = d.t_star * d.t_star;   // exact: t_star is a float
 #pragma unroll
@@ -93,7 +93,7 @@ __device__ void stats_emit(const DevScan& d, const TraitK& tk, int y, int s, con
       // roots and the division are only needed for what is kept
       const double se2 = s2 * vxh;
       if (!(dense || d.store_all || (se2 >= 0.0 && b[j] * b[j] > tt * se2))) continue;
-      const double se = sig * sqrt(vxh * vf);
+      const double se = sqrt(sg2) * sqrt(vxh * vf);
       const double tv = fabs(b[j] / se);
       if (dense) {
         d.d_se[pair * p + h] = se;
@@ -125,7 +125,7 @@ __device__ void stats_emit(const DevScan& d, const TraitK& tk, int y, int s, con
     const int j = jj < R ? LB + R - 1 - jj : LB - 1 - (jj - R);
     if (!((mask >> j) & 1ull)) continue;
     const int h = j < LB ? j : l + (j - LB);
-    const double se = sig * sqrt(vx[h] * vf);
+    const double se = sqrt(sg2) * sqrt(vx[h] * vf);
     const double tv = fabs(b[j] / se);
     oagpu_hit hr;
     hr.trait = d.trait_offset + y;
@@ -139,6 +139,12 @@ __device__ void stats_emit(const DevScan& d, const TraitK& tk, int y, int s, con
     hr.r2 = R2;
     d.hits[o++] = hr;
   }
+}
+
+template <int R, int LB>
+__device__ __forceinline__ void stats_emit(const DevScan& d, const TraitK& tk, int y, int s, const double* b,
+                                           double rss) {
+  stats_emit<R, LB>(d, tk, y, s, b, rss, static_cast<const double*>(d.varx + (size_t)s * d.p));
 }
 
 // RSS over rows with y0 != 0 (src/Algorithm.cpp:976-1024), one thread.
@@ -173,11 +179,11 @@ __device__ double explicit_rss_thread(const DevScan& d, int y, int s, const doub
 
 // Shared end of a pair: the hand-over to the explicit residual pass and the
 // statistics.  b in the split layout (b[0..LB) covariates, b[LB..LB+R) SNP
-// terms).
-template <int R, int LB>
+// terms); yzero and guard (RSS_GUARD * yy) are the trait's.
+template <int R, int LB, typename VX>
 __device__ __forceinline__ void finish_pair(const DevScan& d, const TraitK& tk, int y, int s, bool ok, double* b,
-                                            const double* rb, double rss) {
-  const bool fallback = !ok || d.yzero[y] > 0 || !(rss >= RSS_GUARD * d.yy[y]);
+                                            const double* rb, double rss, bool yzero, double guard, VX vx) {
+  const bool fallback = !ok || yzero || !(rss >= guard);
   if (fallback) {
     if (!ok) {   // sposv left the right-hand side in place
 #pragma unroll
@@ -203,7 +209,14 @@ __device__ __forceinline__ void finish_pair(const DevScan& d, const TraitK& tk, 
     }
     rss = explicit_rss_thread<R, LB>(d, y, s, b);
   }
-  stats_emit<R, LB>(d, tk, y, s, b, rss);
+  stats_emit<R, LB>(d, tk, y, s, b, rss, vx);
+}
+
+template <int R, int LB>
+__device__ __forceinline__ void finish_pair(const DevScan& d, const TraitK& tk, int y, int s, bool ok, double* b,
+                                            const double* rb, double rss) {
+  finish_pair<R, LB>(d, tk, y, s, ok, b, rb, rss, d.yzero[y] > 0, RSS_GUARD * d.yy[y],
+                     static_cast<const double*>(d.varx + (size_t)s * d.p));
 }
 
 }  // namespace oa

@@ -137,3 +137,75 @@ def precision_report(g, o):
     # t straddling the reference's p-rule switch: held to |dt| instead
     assert dt_sw.max(initial=0) <= 1e-3, ("t at the p-rule switch", dt_sw.max())
     return float(db.max(initial=0)), float(dl.max(initial=0))
+
+
+# ---------------------------------------------------------------------------
+# hit records (non-dense scans) against the oracle
+
+MIXED_TOL = 1e-3
+STRONG_P = 1e-8
+
+
+def oracle_records(o, snp0=0):
+    """Kept coefficients of emitted (trait, SNP) records of a dense oracle
+    result, in the reference's order (trait, SNP, h descending)."""
+    kept = o["kept"].astype(bool) & o["emitted"].astype(bool)[:, :, None]
+    y, s, h = np.nonzero(kept)
+    order = np.lexsort((-h, s, y))
+    y, s, h = y[order], s[order], h[order]
+    return {"trait": y, "snp": s + snp0, "h": h, "b": o["beta"][y, s, h], "se": o["se"][y, s, h],
+            "t": o["tstat"][y, s, h], "p": o["pval"][y, s, h], "n_used": o["n_used"][y]}
+
+
+def window(hits, snp_lo, snp_hi, traits=None):
+    sel = (hits["snp"] >= snp_lo) & (hits["snp"] < snp_hi)
+    if traits is not None:
+        sel &= hits["trait"] < traits
+    return hits[sel]
+
+
+def keys(rec):
+    return list(zip(np.asarray(rec["trait"]).tolist(), np.asarray(rec["snp"]).tolist(),
+                    np.asarray(rec["h"]).tolist()))
+
+
+def assert_fp64_records(hits, rec, tol=1e-10):
+    """GPU hits (FP64 path) equal the oracle's kept records."""
+    assert len(hits) == len(rec["trait"]), (len(hits), len(rec["trait"]))
+    assert keys(hits) == keys(rec)
+    np.testing.assert_array_equal(hits["n_used"], rec["n_used"])
+    scale = np.maximum(np.abs(rec["b"]), rec["se"])
+    assert np.all(np.abs(hits["b"] - rec["b"]) <= tol * scale), np.max(np.abs(hits["b"] - rec["b"]) / scale)
+    assert np.all(np.abs(hits["se"] - rec["se"]) <= tol * rec["se"])
+    normal = rec["p"] > 1e-300
+    assert dlog10p(hits["p"][normal], rec["p"][normal]).max(initial=0) <= 1e-8
+    # relative to -log10 p, as in compare_fp64 (near-perfect fits: t ~ 1e7)
+    dlt, dt_sw = dlog10p_t(hits["t"], rec["t"])
+    same_side = (np.abs(hits["t"]) < T_SWITCH) == (np.abs(rec["t"]) < T_SWITCH)
+    lim = 1e-8 + 4.0 * tol * mlog10p_t(rec["t"])[same_side]
+    assert np.all(dlt <= lim), ("-log10 p from t", dlt.max(initial=0))
+    assert dt_sw.max(initial=0) <= 1e-9, ("t at the p-rule switch", dt_sw.max())
+
+
+def mixed_report(hits, rec):
+    """max |dbeta|/SE and max |d -log10 p| over the records both sides kept,
+    and whether the p < 1e-8 sets agree."""
+    idx = {k: i for i, k in enumerate(keys(rec))}
+    pairs = [(j, idx[k]) for j, k in enumerate(keys(hits)) if k in idx]
+    db = dl = 0.0
+    if pairs:
+        jj, ii = (np.array(v) for v in zip(*pairs))
+        db = float(np.max(np.abs(hits["b"][jj] - rec["b"][ii]) / rec["se"][ii]))
+        dls, dt_sw = dlog10p_t(hits["t"][jj], rec["t"][ii])
+        assert dt_sw.max(initial=0) <= MIXED_TOL, ("t at the p-rule switch", dt_sw.max())
+        dl = float(dls.max(initial=0))
+    strong_g = {k for k, p in zip(keys(hits), hits["p"]) if p < STRONG_P}
+    strong_o = {k for k, p in zip(keys(rec), rec["p"]) if p < STRONG_P}
+    return db, dl, strong_g == strong_o, len(strong_o)
+
+
+def assert_mixed_records(hits, rec, label):
+    db, dl, same, ns = mixed_report(hits, rec)
+    print(label, "max|db|/SE %.2e max|d-log10p| %.2e strong hits %d same %s" % (db, dl, ns, same), flush=True)
+    assert db <= MIXED_TOL and dl <= MIXED_TOL, (db, dl)
+    assert same
