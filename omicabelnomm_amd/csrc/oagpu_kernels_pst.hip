@@ -263,6 +263,18 @@ struct PstTrait {
   double zt[LB];
 };
 
+template <int LB>
+__device__ __forceinline__ PstTrait<LB> pst_trait(const DevScan& d, int y) {
+  PstTrait<LB> tr;
+  tr.tk = trait_k(d, y);
+  tr.rss0 = d.rss0[y];
+  tr.guard = RSS_GUARD * d.yy[y];
+  tr.yzero = d.yzero[y] > 0;
+#pragma unroll
+  for (int j = 0; j < LB; ++j) tr.zt[j] = (j < d.l) ? d.zt[(size_t)j * d.ldt + y] : 0.0;
+  return tr;
+}
+
 // One pair, everything included: covariate coefficients when they are
 // reported or the pair is re-evaluated, the hand-over to the explicit
 // residual pass, the statistics and the records.
@@ -303,14 +315,16 @@ using KernDev = const __attribute__((address_space(4))) DevScan;
 
 // The full pair as a call of its own, so that its registers (the explicit
 // residual, the statistics, the records) do not weigh on the first pass.
-// dp is the kernel's DevScan argument in the kernel-argument segment; s and,
-// with UNI, g are wave-uniform.
-template <int R, int LB, typename CT, bool UNI>
-__device__ __noinline__ void pst_pair_cold(KernDev* dp, PstTrait<LB> tr, int y, int g, int s) {
+// dp is the kernel's DevScan argument in the kernel-argument segment; s is
+// wave-uniform.  The trait's constants are loaded again here (rare) rather
+// than handed over, which would put them in scratch memory, and the table
+// row is read per lane (a wave-uniform row in scalar registers does not fit
+// beside the full pair's state for the larger R, LB).
+template <int R, int LB, typename CT>
+__device__ __noinline__ void pst_pair_cold(KernDev* dp, int y, int g, int s) {
   const DevScan d = *(const DevScan*)dp;
   s = __builtin_amdgcn_readfirstlane(s);
-  if (UNI) g = __builtin_amdgcn_readfirstlane(g);
-  pst_pair_full<R, LB, CT, UNI>(d, tr, y, g, s);
+  pst_pair_full<R, LB, CT, false>(d, pst_trait<LB>(d, y), y, g, s);
 }
 
 // The pairs of one trait over a run of SNPs [s_beg, s_end), in one thread.
@@ -328,15 +342,9 @@ __device__ __forceinline__ void pst_trait_run(const DevScan& d, KernDev* dp, int
   using T = PatSnp<R, LB>;
   using RowP = pst_ptr<double, UNI>;
   const int l = d.l;
-  PstTrait<LB> tr;
-  tr.tk = trait_k(d, y);
-  tr.rss0 = d.rss0[y];
-  tr.guard = RSS_GUARD * d.yy[y];
-  tr.yzero = d.yzero[y] > 0;
-#pragma unroll
-  for (int j = 0; j < LB; ++j) tr.zt[j] = (j < l) ? d.zt[(size_t)j * d.ldt + y] : 0.0;
+  const PstTrait<LB> tr = pst_trait<LB>(d, y);
   if (d.need_bl || d.store_all || d.d_beta) {
-    for (int s = s_beg; s < s_end; ++s) pst_pair_cold<R, LB, CT, UNI>(dp, tr, y, g, s);
+    for (int s = s_beg; s < s_end; ++s) pst_pair_cold<R, LB, CT>(dp, y, g, s);
     return;
   }
   const CT* C = reinterpret_cast<const CT*>(d.Cbuf) + y;
@@ -389,7 +397,7 @@ __device__ __forceinline__ void pst_trait_run(const DevScan& d, KernDev* dp, int
   while (pend) {
     const int k = __builtin_ctzll(pend);
     pend &= pend - 1;
-    pst_pair_cold<R, LB, CT, UNI>(dp, tr, y, g, s_beg + k);
+    pst_pair_cold<R, LB, CT>(dp, y, g, s_beg + k);
   }
 }
 
