@@ -12,9 +12,12 @@ the compacted hit list copied back and sorted on the host.  Inputs are
 resident in HBM when the timed region starts.
 
 One JSON line is printed: the mixed-precision (split-BF16 x3) value, the
-FP64 path beside it, the roofline of the dominant kernel (the contraction)
-and the FP64 CPU restatement timed on a bounded sample of the same
-workload.
+FP64 path beside it, the roofline of the dominant kernel (the contraction),
+the FP64 CPU restatement timed on a bounded sample of the same workload,
+the timed runs' hits on that sample checked against the restatement's
+(`oracle_check`; the bench exits with status 3 when they disagree), and
+the latency half of the metric: p50/p99 of a bounded request stream
+(config-4 requests on a 100,000-SNP float16 panel at 20 requests/s).
 """
 from __future__ import annotations
 
@@ -53,6 +56,10 @@ def parse():
     ap.add_argument("--t", type=int, default=0, help="override trait count (testing only)")
     ap.add_argument("--p-store", type=float, default=1e-6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-leg-m", type=int, default=100_000, metavar="SNPS",
+                    help="configs 0-2: panel SNPs of the bounded request-stream leg (0 = skip it)")
+    ap.add_argument("--serve-leg-jobs", type=int, default=32)
+    ap.add_argument("--serve-leg-rate", type=float, default=20.0, help="offered requests/s of that leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--host-fed", type=int, default=0, metavar="BLOCKS",
                     help="also time the scan with genotypes in page-locked host memory, uploaded in BLOCKS "
@@ -197,32 +204,137 @@ def measured_traffic(variant, config):
     return per_launch * rec["launches_per_contraction"]
 
 
-def cpu_baseline(data, budget_s):
-    """FP64 CPU restatement (oracle) on the first m' SNPs x all traits."""
+def cpu_baseline(data, budget_s, p_store):
+    """FP64 CPU restatement (oracle) on m' SNPs x all traits: the SNPs that
+    carry planted effects, then the first SNPs.  Returns the timing record
+    and (the sample's SNP indices, the oracle's dense result on them)."""
     from oracle import oracle as O
     threads = min(16, os.cpu_count() or 1)   # the host's CPU share beside one GPU
     XL = data.XL
     Y = data.y_matrix()
+    planted = np.unique(data.planted[:, 0]) if data.planted is not None else np.zeros(0, dtype=np.int64)
 
-    def run(ms):
-        XR = data.xr_matrix(np.arange(ms))
+    def sample(ms):
+        rest = np.setdiff1d(np.arange(min(data.m, ms + len(planted))), planted)
+        return np.sort(np.concatenate([planted, rest])[:ms])
+
+    def run(snps):
+        XR = data.xr_matrix(snps)
         t0 = time.perf_counter()
-        O.scan(XL, XR, Y, data.r, min_p_store=1e-6, min_p_disp=1e-6, min_r2_store=-10.0,
-               threads=threads)
-        return time.perf_counter() - t0
+        o = O.scan(XL, XR, Y, data.r, min_p_store=p_store, min_p_disp=p_store, min_r2_store=-10.0,
+                   threads=threads)
+        return time.perf_counter() - t0, o
 
     probe = max(1, min(threads, data.m))
-    dt = run(probe)
+    dt, _ = run(np.arange(probe))
     ms = int(min(data.m, max(probe, probe * budget_s / max(dt, 1e-3))))
     ms = max(probe, (ms // threads) * threads) if ms >= threads else ms
-    dt = run(ms)
-    return {"value": ms * data.t / dt, "unit": "SNPxtrait tests/s", "cores": threads, "kind": "port",
-            "sample": "first %d SNPs x %d traits of the same workload (FP64 restatement, OpenMP over SNPs)"
-                      % (ms, data.t), "seconds": round(dt, 3)}
+    snps = sample(ms)
+    dt, o = run(snps)
+    return {"value": len(snps) * data.t / dt, "unit": "SNPxtrait tests/s", "cores": threads, "kind": "port",
+            "sample": "%d SNPs (the %d with planted effects, then the first ones) x %d traits of the same "
+                      "workload (FP64 restatement, OpenMP over SNPs)" % (len(snps), len(planted), data.t),
+            "seconds": round(dt, 3)}, (snps, o)
+
+
+def oracle_check(o, snps, hits, fp64_hits):
+    """The timed runs' hits on the oracle's SNPs against the oracle: the FP64
+    path record for record (beta and SE within 1e-10 relative), the mixed
+    path on the records both keep (max |dbeta|/SE, max |d -log10 p|, both
+    <= 1e-3) and its p < 1e-8 set identical."""
+    kept = o["kept"].astype(bool) & o["emitted"].astype(bool)[:, :, None]
+    y, s, h = np.nonzero(kept)
+    rec = {(int(a), int(snps[b]), int(c)): (o["beta"][a, b, c], o["se"][a, b, c], o["pval"][a, b, c])
+           for a, b, c in zip(y, s, h)}
+
+    def window(hh):
+        return hh[np.isin(hh["snp"], snps)]
+
+    def mlog10(p):
+        return -np.log10(max(float(p), 1e-300))
+
+    out = {"vs": "FP64 oracle (CPU) on the %d sampled SNPs x all traits, hits stored at the bench thresholds"
+                 % len(snps),
+           "n_oracle_records": len(rec)}
+    ok = True
+    if fp64_hits is not None:
+        w = window(fp64_hits)
+        keys = {(int(x["trait"]), int(x["snp"]), int(x["h"])) for x in w}
+        same = keys == set(rec)
+        rb = max((abs(x["b"] - rec[k][0]) / max(abs(rec[k][0]), rec[k][1])
+                  for x in w for k in [(int(x["trait"]), int(x["snp"]), int(x["h"]))] if k in rec), default=0.0)
+        out["fp64_records_identical"] = same
+        out["fp64_max_rel_beta"] = float(rb)
+        ok = ok and same and rb <= 1e-10
+    w = window(hits)
+    db = dl = 0.0
+    for x in w:
+        k = (int(x["trait"]), int(x["snp"]), int(x["h"]))
+        if k in rec:
+            b, se, pv = rec[k]
+            db = max(db, abs(x["b"] - b) / se)
+            dl = max(dl, abs(mlog10(x["p"]) - mlog10(pv)))
+    strong_g = {(int(x["trait"]), int(x["snp"]), int(x["h"])) for x in w if x["p"] < 1e-8}
+    strong_o = {k for k, v in rec.items() if v[2] < 1e-8}
+    out.update({"max_dbeta_over_se": float(db), "max_abs_dlog10p": float(dl),
+                "hits_p_lt_1e-8_identical": strong_g == strong_o, "n_hits_p_lt_1e-8": len(strong_o)})
+    ok = ok and db <= 1e-3 and dl <= 1e-3 and strong_g == strong_o
+    out["ok"] = bool(ok)
+    return out
 
 
 def percentile(xs, q):
     return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if xs else None
+
+
+def serve_leg(m, jobs, rate, p_store, precision):
+    """The latency half of the metric beside the throughput headline: the
+    request workload of config 4 (float16 panel of 20,000 individuals,
+    requests with 1-8,192 traits, 2-16 covariates, MCAR 0-5 %) on a panel of
+    m SNPs, its first `jobs` requests replayed as a Poisson stream at `rate`
+    requests/s, latency = submission to answer."""
+    import threading
+    import torch
+    from omicabelnomm_amd import synth
+    from omicabelnomm_amd.serve import Panel, Server
+    n = synth.SERVE_CONFIGS[4]["n"]
+    specs = synth.job_specs(4, jobs=jobs)
+    G = synth.panel(4, m=m)
+    torch.cuda.synchronize()
+    arrays = [sp.arrays(n) for sp in specs]
+    thr = dict(min_p_store=p_store, min_p_disp=p_store, min_r2_store=-10.0)
+    panel = Panel(G)
+    # the specs' arrivals are a unit-rate Poisson process: scale to `rate`
+    base = synth.SERVE_CONFIGS[4]["rate"]
+    arrivals = [sp.arrival * base / rate for sp in specs]
+    with Server(panel, precision, chunk_snps=8192) as srv:
+        warm = synth.JobSpec(t=64, l=3, keep=None, miss_rate=0.02, arrival=0.0, seed=1)
+        srv.wait(srv.submit(*warm.arrays(n), **thr))   # autotuner, code objects
+        ids = [None] * len(specs)
+        start = time.perf_counter()
+
+        def submitter():
+            for i, sp in enumerate(specs):
+                delay = start + arrivals[i] - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                ids[i] = srv.submit(arrays[i][0], arrays[i][1], sp.keep, max_hits=1 << 22, **thr)
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        th.join()
+        answers = [srv.wait(i) for i in ids]
+        wall = time.perf_counter() - start
+    panel.close()
+    del G
+    lat = [a.latency_ms for a in answers]
+    tests = sum(sp.t for sp in specs) * m
+    return {"workload": "config-4 requests on a float16 panel of %d individuals x %d SNPs resident in HBM"
+                        % (n, m),
+            "requests": len(specs), "traits": sum(sp.t for sp in specs), "offered_rate_per_s": rate,
+            "seconds": round(wall, 3), "tests_per_s": tests / wall,
+            "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99), "max_ms": max(lat),
+            "precision": precision}
 
 
 def bench_serve(args):
@@ -348,6 +460,8 @@ def bench_serve(args):
         "fp64": fp64,
         "speedup_vs_fp64": (value / fp64["value"]) if fp64 else None,
         "precision_check": prec,
+        "oracle_check": ocheck,
+        "latency": serving,
     }
     print(json.dumps(line))
 
@@ -397,6 +511,7 @@ def main():
 
     fp64 = None
     prec = None
+    hits64 = None
     if args.fp64_steps > 0 and args.precision != "fp64":
         sc = make_scanner(data, "fp64", args.p_store)
         dt64, res64, gemm64_ms, chunks64 = timed(sc, data, args.fp64_steps, 1)
@@ -405,6 +520,7 @@ def main():
         fp64 = {"value": tests / dt64, "ms_per_step": dt64 * 1e3,
                 "gemm_tflops": (flops / chunks64) / (gemm64_ms * 1e-3) / 1e12, "gemm_variant": variant64,
                 "timings_ms": res64.timings_ms}
+        hits64 = res64.hits
         # hit set at p < 1e-8 and the -log10 p / beta agreement on shared hits
         key = lambda h: set(zip(h["trait"][h["p"] < 1e-8], h["snp"][h["p"] < 1e-8], h["h"][h["p"] < 1e-8]))
         a, b = key(hits_mixed), key(res64.hits)
@@ -420,11 +536,21 @@ def main():
                 "max_abs_dlog10p": dl, "max_dbeta_over_se": db}
 
     hostfed = host_fed(data, args.precision, args.p_store, args.host_fed, args.steps) if args.host_fed else None
+    serving = None
+    if args.serve_leg_m > 0:
+        serving = serve_leg(args.serve_leg_m, args.serve_leg_jobs, args.serve_leg_rate, args.p_store,
+                            args.precision)
+        print("request stream: p50 %.0f ms p99 %.0f ms" % (serving["p50_ms"], serving["p99_ms"]),
+              file=sys.stderr, flush=True)
     dosage = dosage_leg(data.n, args.dosage) if args.dosage else None
 
     cpu = None
+    ocheck = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(data, args.cpu_seconds)
+        cpu, (snps_o, o) = cpu_baseline(data, args.cpu_seconds, args.p_store)
+        if args.precision == "fp64":
+            hits64, hits_mixed = hits_mixed, hits_mixed
+        ocheck = oracle_check(o, snps_o, hits_mixed, hits64)
 
     peak = PEAK[args.precision]
     line = {
@@ -453,6 +579,8 @@ def main():
         "fp64": fp64,
         "speedup_vs_fp64": (value / fp64["value"]) if fp64 else None,
         "precision_check": prec,
+        "oracle_check": ocheck,
+        "latency": serving,
         "timings_ms": timings,
         "stats": {k: int(v) if isinstance(v, (int, np.integer)) else v for k, v in res.stats.items()},
     }
@@ -461,6 +589,9 @@ def main():
     if dosage:
         line["dosage_combine"] = dosage
     print(json.dumps(line))
+    if ocheck is not None and not ocheck["ok"]:
+        print("oracle check failed: %s" % json.dumps(ocheck), file=sys.stderr)
+        sys.exit(3)
 
 
 if __name__ == "__main__":

@@ -19,7 +19,9 @@
  * the chosen contraction kernel).
  *
  * Requests whose individual subset is not everyone need a panel without
- * missing genotypes (a missing genotype is imputed with the panel-wide mean).
+ * missing genotypes: the shared operand pass imputes a missing genotype with
+ * the panel-wide mean, where a scan of the subset uses the subset's mean, so
+ * oagpu_server_submit refuses such a request (OAGPU_ERR_UNSUPPORTED).
  */
 #ifndef OAGPU_SERVE_H
 #define OAGPU_SERVE_H
@@ -68,7 +70,9 @@ int oagpu_server_create(oagpu_panel *panel, const oagpu_server_params *params, o
 void oagpu_server_destroy(oagpu_server *server);
 const char *oagpu_server_last_error(const oagpu_server *server);
 
-/* Copies the request's inputs and queues it; returns at once. */
+/* Copies the request's inputs and queues it; returns at once.
+ * OAGPU_ERR_UNSUPPORTED: a subset (keep with a zero flag) on a panel that
+ * holds missing genotypes. */
 int oagpu_server_submit(oagpu_server *server, const oagpu_request *req, long long *id);
 /* 1 when the request has finished, 0 while it is queued or running, < 0 on
  * an unknown id. */

@@ -555,6 +555,12 @@ extern "C" int oagpu_server_submit(oagpu_server* S, const oagpu_request* q, long
     S->err = "bad request (t >= 1, 1 <= l <= 32, ldy and ldxl >= n)";
     return OAGPU_ERR_ARG;
   }
+  if (q->keep && S->panel->has_nan && std::find(q->keep, q->keep + n, (uint8_t)0) != q->keep + n) {
+    // the shared operand pass imputes with the panel-wide column means; the
+    // subset's own means would be needed
+    S->err = "an individual subset needs a panel without missing genotypes";
+    return OAGPU_ERR_UNSUPPORTED;
+  }
   auto* R = new Request();
   R->t = q->t;
   R->l = q->l;

@@ -1,6 +1,15 @@
 """Readers of the scan's output files, following OmicOUT's parser of
 _sto.ibin / _sto.dbin (OmicOUT/main.cpp:280-461) and the _dis.txt layout
-(src/AIOwrapper.cpp:945,1497-1511).  Test helpers."""
+(src/AIOwrapper.cpp:945,1497-1511).  Test helpers.
+
+Two _sto.dbin layouts (src/AIOwrapper.cpp:1454-1531):
+- default: per (trait block, SNP block) a header (first trait, record
+  count, AR offset), then records of n_used, R2 (float16), size_p and
+  size_p results (AL index when covariates are displayed, AR index, B, SE);
+- -f (storePInd): no header, no size_p and no indices; every (trait, SNP)
+  record holds n_used, R2 and h_max = r (+ l - 1 with -i) results (B, SE),
+  h descending, so a record's position says what it is
+  (OmicOUT/main.cpp:347-420)."""
 import struct
 
 import numpy as np
@@ -33,7 +42,7 @@ def read_dbin(path, disp_cov, store_all, l, r, max_blocks=None):
     blocks = 0
     while pos < len(data) and (max_blocks is None or blocks < max_blocks):
         if store_all:
-            raise NotImplementedError("-f layout is positional; not parsed here")
+            raise ValueError("the -f layout is positional: read it with read_dbin_pind")
         y_idx, count, ar_off = struct.unpack_from("<3i", data, pos)
         pos += 12
         for _ in range(count):
@@ -118,4 +127,32 @@ def expected_records(o, l, r, tb=1000, mb=500):
                             res.append((255 if h >= l else h, ii * r + (h - l if h >= l else 0),
                                         o["beta"][y, s, h], o["se"][y, s, h]))
                     out.append((first, s0 * r, int(o["n_used"][y]), res))
+    return out
+
+
+def read_dbin_pind(path, h_max):
+    """The -f (storePInd) layout: fixed-size records in file order, each
+    dict(n_used, r2, results=[(B, SE)] * h_max)."""
+    data = open(path, "rb").read()
+    size = 4 + 8 * h_max
+    assert len(data) % size == 0, (len(data), size)
+    recs = []
+    for pos in range(0, len(data), size):
+        n_used, r2b = struct.unpack_from("<HH", data, pos)
+        vals = struct.unpack_from("<%df" % (2 * h_max), data, pos + 4)
+        recs.append(dict(n_used=n_used, r2=float32_from_bits16(r2b),
+                         results=[(vals[2 * k], vals[2 * k + 1]) for k in range(h_max)]))
+    return recs
+
+
+def pind_order(t, m, tb=1000, mb=500):
+    """(trait, SNP) of each -f record in the writer's order: trait block, SNP
+    block, trait, SNP.  OmicOUT reads them as trait-major over every SNP,
+    which is the same order when t <= tb and m <= mb."""
+    out = []
+    for y0 in range(0, t, tb):
+        for s0 in range(0, m, mb):
+            for y in range(y0, min(t, y0 + tb)):
+                for s in range(s0, min(m, s0 + mb)):
+                    out.append((y, s))
     return out

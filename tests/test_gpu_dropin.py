@@ -11,7 +11,8 @@ import numpy as np
 import pytest
 
 from omicabelnomm_amd import databel
-from omicout import expected_dis_rows, expected_records, read_dbin, read_dis, read_ibin
+from omicout import (expected_dis_rows, expected_records, pind_order, read_dbin, read_dbin_pind, read_dis,
+                     read_ibin)
 from parity_util import compare_fp64
 from refmodel import CHECK_SETTINGS, GOLDEN, load_fixture, prepare, scenarios
 
@@ -122,6 +123,55 @@ def test_dropin_files_basic_model(oracle, fx, tmp_path):
     assert "Number of Significant Results: %d" % o["n_sig"] in p.stdout
     assert open(os.path.join(tmp, "cli_dis.txt")).read() == open(os.path.join(tmp, "basic_dis.txt")).read()
     assert open(os.path.join(tmp, "cli_sto.dbin"), "rb").read() == open(os.path.join(tmp, "basic_sto.dbin"), "rb").read()
+
+
+def test_dropin_files_store_pind(oracle, fx, tmp_path):
+    """-f (storePInd) with -b and -i: every coefficient of every (trait, SNP)
+    pair is stored, in the positional layout (src/AIOwrapper.cpp:1454-1531:
+    no block header, no size_p, no AL/AR index), and every one of them is a
+    _dis.txt row.  The records are read back as OmicOUT reads them
+    (OmicOUT/main.cpp:347-420) and compared with the oracle's store-all scan;
+    the CLI's -f files are byte-equal to Algorithm.solve's."""
+    from omicabelnomm_amd import Algorithm, Outputs
+    tmp = str(tmp_path)
+    _write_inputs(fx, tmp)
+    s = _settings(tmp, "pind")
+    s.storePInd = True
+    out = Outputs()
+    Algorithm().solve(s, out)
+    excl = open(os.path.join(GOLDEN, "exclude_individuals.txt")).read()
+    d = prepare(fx, None, excl_text=excl, limit_t=50, limit_m=100)
+    o = oracle.scan(d["XL"], d["XR"], d["Y"], 1, real_n=d["realN"], min_p_store=0.00005,
+                    min_p_disp=0.00005, min_r2_store=0.000001, disp_cov=True, store_all=True, threads=4)
+    t, m, l, r = 50, 100, 4, 1
+    assert o["emitted"].all()                       # every pair is a record
+    ib = read_ibin(os.path.join(tmp, "pind_sto.ibin"))
+    assert ib["store_all"] and ib["disp_cov"]
+    h_max = r + l - 1                                # the intercept is not stored
+    recs = read_dbin_pind(os.path.join(tmp, "pind_sto.dbin"), h_max)
+    order = pind_order(t, m)
+    assert len(recs) == len(order) == t * m
+    for rc, (y, snp) in zip(recs, order):
+        assert rc["n_used"] == o["n_used"][y]
+        assert rc["r2"] == pytest.approx(float(np.float32(o["r2"][y, snp])), rel=2 ** -7, abs=1e-6)
+        want = [(o["beta"][y, snp, h], o["se"][y, snp, h]) for h in range(l + r - 1, 0, -1)]
+        for (B, SE), (wB, wSE) in zip(rc["results"], want):
+            assert B == pytest.approx(np.float32(wB), rel=1e-6, abs=1e-7)
+            assert SE == pytest.approx(np.float32(wSE), rel=1e-6, abs=1e-7)
+    rows = read_dis(os.path.join(tmp, "pind_dis.txt"))
+    names = dict(y=list(fx["y_names"]), al=list(fx["xl_names"]), ar=list(fx["xr_names"]))
+    want_rows = expected_dis_rows(o, names, l, r, 0.00005, 0.000001, d["realN"], store_all=True)
+    assert len(rows) == len(want_rows) == t * m * h_max
+    assert [(x["phe"], x["snp"]) for x in rows] == [(w["phe"], w["snp"]) for w in want_rows]
+    cli = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "omicabelnomm_amd", "omicabelnomm")
+    cmd = [cli, "-c", os.path.join(tmp, "XL.fvi"), "-g", os.path.join(tmp, "XR.fvi"), "-p",
+           os.path.join(tmp, "Y.fvi"), "-o", os.path.join(tmp, "cli.txt"), "-x", os.path.join(tmp, "excl.txt"),
+           "-d", "0.00005", "-s", "0.1", "-e", "0.00001", "-r", "0.000001", "-i", "-b", "-f"]
+    p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout
+    for suffix in ("_dis.txt", "_sto.dbin", "_sto.ibin"):
+        assert open(os.path.join(tmp, "cli" + suffix), "rb").read() == \
+            open(os.path.join(tmp, "pind" + suffix), "rb").read(), suffix
 
 
 def test_dropin_files_additive_dosage(oracle, fx, tmp_path):

@@ -170,3 +170,27 @@ def test_panel_with_missing_genotypes(precision):
     for (Y, XL), a in zip(reqs, answers):
         assert_hits_equal(a.hits, scan_alone(G, Y, XL, None, precision, **THR), 1e-12 if precision == "fp64" else 1e-4)
     panel.close()
+
+
+def test_subset_refused_on_a_panel_with_missing_genotypes():
+    """The shared operand pass imputes a missing genotype with the
+    panel-wide mean, where a scan of a subset uses the subset's mean: a
+    request with an individual subset on such a panel is refused
+    (OAGPU_ERR_UNSUPPORTED), one with every flag set is served."""
+    from omicabelnomm_amd import lib as L
+    d = synth.make(n=300, l=2, m=40, t=2, miss="none", geno_nan=0.02, seed=46)
+    G = d.G.contiguous()
+    n = G.shape[1]
+    Y, XL = request_data(n, 6, 3, "mcar", 33)
+    keep = np.ones(n, dtype=np.uint8)
+    keep[::7] = 0
+    print("case nan panel subset", flush=True)
+    panel = Panel(G.to("cuda"))
+    with Server(panel, "fp64", chunk_snps=64) as srv:
+        with pytest.raises(L.OagpuError) as e:
+            srv.submit(Y, XL, keep, **THR)
+        assert e.value.code == L.ERR_UNSUPPORTED
+        assert "subset" in str(e.value)
+        a = srv.wait(srv.submit(Y, XL, np.ones(n, dtype=np.uint8), store_all=True, **THR))
+    assert_hits_equal(a.hits, scan_alone(G, Y, XL, None, "fp64", **THR), 1e-12)
+    panel.close()
