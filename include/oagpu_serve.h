@@ -47,6 +47,9 @@ typedef struct oagpu_server_params {
     long long hbm_budget;   /* bytes for active requests (0 = 64 GiB)               */
     int max_active;         /* requests in one batch (0 = 64)                       */
     int chunk_snps;         /* SNPs per chunk (0 = 16384)                           */
+    int no_graphs;          /* 1: launch each chunk's kernels one by one; 0: replay
+                               them from a HIP graph captured once per active set
+                               and chunk size                                       */
 } oagpu_server_params;
 
 typedef struct oagpu_request {
@@ -69,6 +72,9 @@ int oagpu_server_create(oagpu_panel *panel, const oagpu_server_params *params, o
  * requests fail with OAGPU_ERR_STATE. */
 void oagpu_server_destroy(oagpu_server *server);
 const char *oagpu_server_last_error(const oagpu_server *server);
+
+/* Chunks this server has replayed from a captured graph so far. */
+long long oagpu_server_graph_chunks(const oagpu_server *server);
 
 /* Copies the request's inputs and queues it; returns at once.
  * OAGPU_ERR_UNSUPPORTED: a subset (keep with a zero flag) on a panel that

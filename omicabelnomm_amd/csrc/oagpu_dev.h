@@ -87,6 +87,10 @@ struct DevScan {
 
   // chunk
   int s0, mc;            // first SNP of the chunk, SNPs in the chunk
+  // when set, the chunk's first SNP is read from here instead of s0: a
+  // launch sequence captured once in a HIP graph is replayed for every chunk
+  // of one size, the start written before each replay
+  const int* s0_dev;
   double* col_mean;      // [mc*r]
   double* Sbr;           // [mc][r(r+1)/2] packed upper
   double* Str;           // [mc][l][r]

@@ -4,6 +4,12 @@
 
 namespace oa {
 
+// A per-chunk kernel's first step: take the chunk start from device memory
+// when the launch belongs to a replayed graph (DevScan::s0_dev).
+__device__ __forceinline__ void chunk_start(DevScan& d) {
+  if (d.s0_dev) d.s0 = *d.s0_dev;
+}
+
 // trait side (oagpu_kernels_trait.hip)
 void launch_ystats(const DevScan& d, int prec, void* Bhi, void* Blo, int* rows_out, int* ymiss,
                    int* yzero, double* yy, double* sst, double* rt, unsigned long long* hash,
@@ -27,7 +33,7 @@ void launch_trait_solve(const DevScan& d, int lb, double* zt, double* ay, double
 void launch_prep(const DevScan& d, int prec, void* Ahi, void* Alo, bool corr, hipStream_t st);
 // Xu[u][s] = G[(s0 + s) * ldg + u] for u < n, s < mc (gdt DT_F32 or DT_F16)
 void launch_panel_rows(const void* G, int gdt, long long ldg, int s0, int mc, int n, void* Xu, long long ldxu,
-                       hipStream_t st);
+                       hipStream_t st, const int* s0_dev = nullptr);
 // Dosage model: file SNP s has file_r columns of which the first n_used are
 // read; sum != 0 writes one column per SNP, sum_i f[i] * G[:, s*file_r+i],
 // otherwise n_used columns f[i] * G[:, s*file_r+i].  FP64 out (n rows,
@@ -52,6 +58,8 @@ void launch_reset_counter(unsigned long long* c, int idx, hipStream_t st);
 // zero bytes (a multiple of 4) at p with a kernel (no memset node when the
 // caller captures the stream)
 void launch_zero(void* p, size_t bytes, hipStream_t st);
+// *p = v, by a kernel (no memset or copy node where the caller captures)
+void launch_set_int(int* p, int v, hipStream_t st);
 // counters [0, n), n <= 64, in one launch
 void launch_reset_counters(unsigned long long* c, int n, hipStream_t st);
 // r = 1: out[s] = sum over the subset's rows (d.keep) of x^2

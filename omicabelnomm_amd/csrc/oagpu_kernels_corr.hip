@@ -61,6 +61,7 @@ __device__ __forceinline__ void fmac_row(double (&lr)[LB][W], double v, const do
 // for: the exact l, or a bucket whose entries past l are zero.
 template <int R, int LB, typename XT>
 __global__ __launch_bounds__(256) void k_corr(DevScan d, int gq_count) {
+  chunk_start(d);
   constexpr int SPL = corr_spl<R, LB>();
   constexpr int W = SPL * R;   // Xu values per lane and row
   typedef typename XVec<XT, W>::T XV;

@@ -17,6 +17,7 @@ namespace oa {
 // serves the tile's contraction rows to all of them.
 template <int R, int LB, typename CT>
 __global__ __launch_bounds__(256) void k_epilogue(DevScan d, int n_q) {
+  chunk_start(d);
   const int lane = threadIdx.x & 63;
   const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
   const int q = local % n_q, tile = (local / n_q) * 8 + xcd;

@@ -83,6 +83,7 @@ __device__ __forceinline__ typename GVec<GT>::T load_g4(const DevScan& d, int sr
 // is a template parameter, which keeps every load free of branches.
 template <int R, int OP, typename XT, typename GT>
 __global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo) {
+  chunk_start(d);
   constexpr bool CORR = !std::is_void<XT>::value;
   constexpr int RR = R * (R + 1) / 2;
   constexpr int PREP_SPW = prep_spw<R>();
@@ -306,7 +307,8 @@ static void prep_r(const DevScan& d, int prec, void* Ahi, void* Alo, bool corr, 
 // would write 8-byte pieces to 4 rows per lane.
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel_rows(const T* __restrict__ G, long long ldg, int s0, int mc, int n,
-                                                    T* __restrict__ Xu, long long ldxu) {
+                                                    T* __restrict__ Xu, long long ldxu, const int* s0_dev) {
+  if (s0_dev) s0 = *s0_dev;   // replayed from a graph
   __shared__ T tile[64][65];
   const int u0 = blockIdx.x * 64, sb = blockIdx.y * 64;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
@@ -322,14 +324,14 @@ __global__ __launch_bounds__(256) void k_panel_rows(const T* __restrict__ G, lon
 }
 
 void launch_panel_rows(const void* G, int gdt, long long ldg, int s0, int mc, int n, void* Xu, long long ldxu,
-                       hipStream_t st) {
+                       hipStream_t st, const int* s0_dev) {
   dim3 grid((n + 63) / 64, (mc + 63) / 64), block(256);
   if (gdt == DT_F32)
     hipLaunchKernelGGL(k_panel_rows<float>, grid, block, 0, st, static_cast<const float*>(G), ldg, s0, mc, n,
-                       static_cast<float*>(Xu), ldxu);
+                       static_cast<float*>(Xu), ldxu, s0_dev);
   else
     hipLaunchKernelGGL(k_panel_rows<_Float16>, grid, block, 0, st, static_cast<const _Float16*>(G), ldg, s0, mc, n,
-                       static_cast<_Float16*>(Xu), ldxu);
+                       static_cast<_Float16*>(Xu), ldxu, s0_dev);
 }
 
 // Dosage model columns, formed once per genotype block before the scan

@@ -32,6 +32,7 @@ struct PatSnp {
 
 template <int R, int LB>
 __global__ __launch_bounds__(256) void k_patsnp(DevScan d) {
+  chunk_start(d);
   using T = PatSnp<R, LB>;
   constexpr int RR = T::RR;
   const int G = d.G, l = d.l;
@@ -315,14 +316,15 @@ using KernDev = const __attribute__((address_space(4))) DevScan;
 
 // The full pair as a call of its own, so that its registers (the explicit
 // residual, the statistics, the records) do not weigh on the first pass.
-// dp is the kernel's DevScan argument in the kernel-argument segment; s is
-// wave-uniform.  The trait's constants are loaded again here (rare) rather
+// dp is the kernel's DevScan argument in the kernel-argument segment, s0
+// the chunk start the kernel uses (DevScan::s0_dev); s is wave-uniform.  The trait's constants are loaded again here (rare) rather
 // than handed over, which would put them in scratch memory, and the table
 // row is read per lane (a wave-uniform row in scalar registers does not fit
 // beside the full pair's state for the larger R, LB).
 template <int R, int LB, typename CT>
-__device__ __noinline__ void pst_pair_cold(KernDev* dp, int y, int g, int s) {
-  const DevScan d = *(const DevScan*)dp;
+__device__ __noinline__ void pst_pair_cold(KernDev* dp, int s0, int y, int g, int s) {
+  DevScan d = *(const DevScan*)dp;
+  d.s0 = __builtin_amdgcn_readfirstlane(s0);   // the kernel's chunk start
   s = __builtin_amdgcn_readfirstlane(s);
   pst_pair_full<R, LB, CT, false>(d, pst_trait<LB>(d, y), y, g, s);
 }
@@ -344,7 +346,7 @@ __device__ __forceinline__ void pst_trait_run(const DevScan& d, KernDev* dp, int
   const int l = d.l;
   const PstTrait<LB> tr = pst_trait<LB>(d, y);
   if (d.need_bl || d.store_all || d.d_beta) {
-    for (int s = s_beg; s < s_end; ++s) pst_pair_cold<R, LB, CT>(dp, y, g, s);
+    for (int s = s_beg; s < s_end; ++s) pst_pair_cold<R, LB, CT>(dp, d.s0, y, g, s);
     return;
   }
   const CT* C = reinterpret_cast<const CT*>(d.Cbuf) + y;
@@ -397,7 +399,7 @@ __device__ __forceinline__ void pst_trait_run(const DevScan& d, KernDev* dp, int
   while (pend) {
     const int k = __builtin_ctzll(pend);
     pend &= pend - 1;
-    pst_pair_cold<R, LB, CT>(dp, y, g, s_beg + k);
+    pst_pair_cold<R, LB, CT>(dp, d.s0, y, g, s_beg + k);
   }
 }
 
@@ -406,6 +408,7 @@ __device__ __forceinline__ void pst_trait_run(const DevScan& d, KernDev* dp, int
 // SNP run are on one XCD and read the run's table rows from its L2.
 template <int R, int LB, typename CT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_epilogue_pst(DevScan d, int n_wb, int run) {
+  chunk_start(d);
   const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
   const int wb = local % n_wb, rq = (local / n_wb) * 8 + xcd;
   const int w = wb * 4 + (int)(threadIdx.x >> 6);

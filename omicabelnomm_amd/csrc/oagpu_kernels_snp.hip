@@ -26,6 +26,7 @@ namespace oa {
 // inverse with U0 = chol(Stl).
 template <int R, int LB, typename CT>
 __global__ void k_snp_gram(DevScan d) {
+  chunk_start(d);
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= d.mc) return;
   const int l = d.l, p = d.p;
@@ -152,6 +153,7 @@ __global__ void k_snp_gram(DevScan d) {
 // per pair.
 template <int R, int LB>
 __global__ __launch_bounds__(256) void k_fallback(DevScan d) {
+  chunk_start(d);
   constexpr int P = LB + R;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   unsigned long long total = d.counters[CNT_FB_CHUNK];
@@ -206,6 +208,7 @@ __global__ void k_reset_counters(unsigned long long* c, int n) {
 // chunk's operand but not its individuals); x is imputed with the chunk's
 // column mean.  One block per SNP.
 __global__ __launch_bounds__(256) void k_subset_sbr(DevScan d, double* out) {
+  chunk_start(d);
   __shared__ double shd[4];
   const int s = blockIdx.x;
   const int gs = d.s0 + s;
@@ -258,6 +261,10 @@ void launch_zero(void* p, size_t bytes, hipStream_t st) {
   const unsigned blocks = (unsigned)std::min<size_t>((words + 255) / 256, 1024);
   hipLaunchKernelGGL(k_zero_words, dim3(blocks), dim3(256), 0, st, static_cast<uint32_t*>(p), words);
 }
+
+__global__ void k_set_int(int* p, int v) { *p = v; }
+
+void launch_set_int(int* p, int v, hipStream_t st) { hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, p, v); }
 
 void launch_reset_counter(unsigned long long* c, int idx, hipStream_t st) {
   hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, st, c, idx);

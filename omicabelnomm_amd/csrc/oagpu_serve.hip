@@ -126,6 +126,17 @@ struct oagpu_server {
   size_t C_cap = 0;
   int ldc = 0;
   bool arena_dirty = true;
+
+  // graphs of a chunk's launches: one per chunk size (slot 0 the full
+  // chunks, slot 1 the last, shorter one), captured for the active set of
+  // epoch `epoch` and replayed with the chunk start written to s0_dev
+  long long epoch = 0;
+  int* s0_dev = nullptr;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  long long g_epoch[2] = {-1, -1};   // active set a slot's graph was captured for
+  long long g_seen[2] = {-1, -1};    // active set a slot last ran eagerly for
+  bool graphs_off = false;           // no_graphs, or a capture failed
+  long long graph_chunks = 0;        // chunks replayed from a graph
 };
 
 #define SRVCK(S, expr)                                                                 \
@@ -280,13 +291,15 @@ int rebuild_arena(oagpu_server* S) {
     SRVCK(S, grow(&S->Cbuf, &S->C_cap, cbytes));
   }
   S->arena_dirty = false;
+  ++S->epoch;   // a new active set: captured graphs no longer apply
   return OAGPU_OK;
 }
 
-int run_chunk(oagpu_server* S) {
+// The launches of one chunk (operand pass, contraction, every active
+// request's pair side on the lane streams, joined back into S->st).  With
+// s0_dev the kernels read the chunk start from there (graph capture).
+int enqueue_chunk(oagpu_server* S, int s0, int mcc, const int* s0_dev) {
   const oagpu_panel* P = S->panel;
-  const int s0 = S->cur * S->mc;
-  const int mcc = (int)std::min<long long>(S->mc, P->m - s0);
   const int prec = S->prm.precision;
   bool any_missing = false;
   for (Request* R : S->active) any_missing = any_missing || R->ctx->compact_corr;
@@ -304,6 +317,7 @@ int run_chunk(oagpu_server* S) {
   dp.ldg = P->ldg;
   dp.g_vec = ((uintptr_t)P->G % 16 == 0) && (P->ldg % 4 == 0);
   dp.s0 = s0;
+  dp.s0_dev = s0_dev;
   dp.mc = mcc;
   dp.col_mean = S->col_mean;
   dp.Sbr = S->Sbr;
@@ -315,7 +329,7 @@ int run_chunk(oagpu_server* S) {
     // no missing genotype in the panel: the compact rows are the chunk itself,
     // transposed, written by a tiled kernel instead of the operand pass
     launch_prep(dp, prec, S->Ahi, S->Alo, false, S->st);
-    launch_panel_rows(P->G, S->xu_dt, P->ldg, s0, mcc, P->n, S->Xu, S->ldxu, S->st);
+    launch_panel_rows(P->G, S->xu_dt, P->ldg, s0, mcc, P->n, S->Xu, S->ldxu, S->st, s0_dev);
   } else {
     launch_prep(dp, prec, S->Ahi, S->Alo, any_missing, S->st);
   }
@@ -342,6 +356,7 @@ int run_chunk(oagpu_server* S) {
     hipStream_t st = lane == 0 ? S->st : S->lane_st[lane - 1];
     DevScan d = make_dev(c);
     d.s0 = s0;
+    d.s0_dev = s0_dev;
     d.mc = mcc;
     d.Gm = P->G;
     d.gdt = P->dtype;
@@ -371,6 +386,78 @@ int run_chunk(oagpu_server* S) {
   for (int w = 0; w < kLanes - 1; ++w) {
     SRVCK(S, hipEventRecord(S->ev_lane[w], S->lane_st[w]));
     SRVCK(S, hipStreamWaitEvent(S->st, S->ev_lane[w], 0));
+  }
+  SRVCK(S, hipGetLastError());
+  return OAGPU_OK;
+}
+
+bool graph_has_memset(hipGraph_t g) {
+  size_t n = 0;
+  if (hipGraphGetNodes(g, nullptr, &n) != hipSuccess) return true;
+  std::vector<hipGraphNode_t> nodes(n);
+  if (n && hipGraphGetNodes(g, nodes.data(), &n) != hipSuccess) return true;
+  for (hipGraphNode_t x : nodes) {
+    hipGraphNodeType ty;
+    if (hipGraphNodeGetType(x, &ty) != hipSuccess || ty == hipGraphNodeTypeMemset) return true;
+  }
+  return false;
+}
+
+// Captures a chunk's launches into a graph of slot `slot`; false (and
+// graphs switched off for this server) when the capture or its
+// instantiation fails or the graph holds a memset node.
+bool capture_chunk(oagpu_server* S, int slot, int s0, int mcc) {
+  if (S->gexec[slot]) {
+    (void)hipGraphExecDestroy(S->gexec[slot]);
+    S->gexec[slot] = nullptr;
+  }
+  if (hipStreamBeginCapture(S->st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    S->graphs_off = true;
+    return false;
+  }
+  const int rc = enqueue_chunk(S, s0, mcc, S->s0_dev);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(S->st, &g);
+  bool ok = rc == OAGPU_OK && e == hipSuccess && g && !graph_has_memset(g);
+  if (ok) ok = hipGraphInstantiate(&S->gexec[slot], g, nullptr, nullptr, 0) == hipSuccess;
+  if (g) (void)hipGraphDestroy(g);
+  (void)hipGetLastError();
+  if (!ok) {
+    S->gexec[slot] = nullptr;
+    S->graphs_off = true;
+    return false;
+  }
+  S->g_epoch[slot] = S->epoch;
+  return true;
+}
+
+// One chunk: replayed from the slot's graph when one was captured for the
+// current active set, else launched one by one.  A slot's first chunk of an
+// active set runs eagerly (it also settles the contraction autotuner, whose
+// timing runs cannot be captured); its second is captured and replayed.
+int run_chunk(oagpu_server* S) {
+  const oagpu_panel* P = S->panel;
+  const int s0 = S->cur * S->mc;
+  const int mcc = (int)std::min<long long>(S->mc, P->m - s0);
+  const int slot = mcc == S->mc ? 0 : 1;
+  bool replay = false;
+  if (!S->graphs_off) {
+    if (S->g_epoch[slot] == S->epoch && S->gexec[slot]) {
+      replay = true;
+    } else if (S->g_seen[slot] == S->epoch) {
+      replay = capture_chunk(S, slot, s0, mcc);
+    } else {
+      S->g_seen[slot] = S->epoch;
+    }
+  }
+  if (replay) {
+    launch_set_int(S->s0_dev, s0, S->st);
+    SRVCK(S, hipGraphLaunch(S->gexec[slot], S->st));
+    std::lock_guard<std::mutex> lk(S->mu);
+    ++S->graph_chunks;
+  } else {
+    const int rc = enqueue_chunk(S, s0, mcc, nullptr);
+    if (rc) return rc;
   }
   SRVCK(S, hipGetLastError());
   SRVCK(S, hipStreamSynchronize(S->st));
@@ -516,6 +603,8 @@ extern "C" int oagpu_server_create(oagpu_panel* P, const oagpu_server_params* pr
   SRVCK(S, hipMemset(static_cast<char*>(S->Xu) + xu_el * (size_t)P->n * S->ldxu, 0, xu_el * S->ldxu));
   std::vector<int> id(P->n_pad, -1);
   for (int k = 0; k < P->n; ++k) id[k] = k;
+  SRVCK(S, hipMalloc(&S->s0_dev, sizeof(int)));
+  S->graphs_off = S->prm.no_graphs != 0;
   SRVCK(S, hipMalloc(&S->uidx, sizeof(int) * id.size()));
   SRVCK(S, hipMemcpy(S->uidx, id.data(), sizeof(int) * id.size(), hipMemcpyHostToDevice));
   S->th = std::thread(server_loop, S);
@@ -535,7 +624,9 @@ extern "C" void oagpu_server_destroy(oagpu_server* S) {
     delete kv.second;
   }
   (void)hipSetDevice(S->panel->device);
-  void* bufs[] = {S->Abuf, S->col_mean, S->Sbr, S->Xu, S->uidx, S->Barena, S->Cbuf};
+  for (hipGraphExec_t& x : S->gexec)
+    if (x) (void)hipGraphExecDestroy(x);
+  void* bufs[] = {S->Abuf, S->col_mean, S->Sbr, S->Xu, S->uidx, S->Barena, S->Cbuf, S->s0_dev};
   for (void* b : bufs) dfree(b);
   for (hipStream_t w : S->lane_st)
     if (w) (void)hipStreamDestroy(w);
@@ -547,6 +638,12 @@ extern "C" void oagpu_server_destroy(oagpu_server* S) {
 }
 
 extern "C" const char* oagpu_server_last_error(const oagpu_server* S) { return S ? S->err.c_str() : "null server"; }
+
+extern "C" long long oagpu_server_graph_chunks(const oagpu_server* S) {
+  if (!S) return 0;
+  std::lock_guard<std::mutex> lk(const_cast<oagpu_server*>(S)->mu);
+  return S->graph_chunks;
+}
 
 extern "C" int oagpu_server_submit(oagpu_server* S, const oagpu_request* q, long long* id) {
   if (!S || !q || !id) return OAGPU_ERR_ARG;

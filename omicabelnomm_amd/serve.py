@@ -50,11 +50,11 @@ class Answer:
 
 
 class Server:
-    def __init__(self, panel: Panel, precision="bf16x3", hbm_budget=0, max_active=0, chunk_snps=0):
+    def __init__(self, panel: Panel, precision="bf16x3", hbm_budget=0, max_active=0, chunk_snps=0, graphs=True):
         self._L = L.load()
         self.panel = panel
         prm = L.ServerParams(precision=L.PRECISIONS[precision], hbm_budget=hbm_budget, max_active=max_active,
-                             chunk_snps=chunk_snps)
+                             chunk_snps=chunk_snps, no_graphs=0 if graphs else 1)
         self._s = ctypes.c_void_p()
         rc = self._L.oagpu_server_create(panel._p, ctypes.byref(prm), ctypes.byref(self._s))
         if rc != L.OK:
@@ -80,6 +80,10 @@ class Server:
             raise L.OagpuError(rc, self._L.oagpu_server_last_error(self._s).decode())
         self._max_hits[rid.value] = max_hits
         return rid.value
+
+    def graph_chunks(self) -> int:
+        """Chunks replayed from a captured HIP graph so far."""
+        return int(self._L.oagpu_server_graph_chunks(self._s))
 
     def done(self, rid) -> bool:
         return self._L.oagpu_server_poll(self._s, rid) == 1

@@ -194,3 +194,44 @@ def test_subset_refused_on_a_panel_with_missing_genotypes():
         a = srv.wait(srv.submit(Y, XL, np.ones(n, dtype=np.uint8), store_all=True, **THR))
     assert_hits_equal(a.hits, scan_alone(G, Y, XL, None, "fp64", **THR), 1e-12)
     panel.close()
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 0.0), ("bf16x3", 1e-4)])
+def test_graph_replay_equals_launches(precision, tol):
+    """A chunk's launches replayed from a captured HIP graph (the chunk start
+    written to device memory before each replay) give the same answers as
+    launching them one by one: two requests over 9 chunks (the first full
+    chunk of an active set runs eagerly, the second is captured, the rest
+    replay it; the short last chunk has its own graph), then a third request
+    alone (a new active set, captured again).  FP64 answers are equal bit for
+    bit; split-BF16 ones within the tolerance of a request served alone or
+    in a mix (when the two requests are admitted in different iterations the
+    active sets differ, and with them the contraction's column count and
+    kernel)."""
+    G = panel_data(m=270, seed=47)
+    n = G.shape[1]
+    reqs = [request_data(n, 24, 3, "mcar", 34), request_data(n, 40, 2, "groups", 35),
+            request_data(n, 9, 4, "mcar", 36)]
+    panel = Panel(G.to("cuda"))
+    out = {}
+    for graphs in (False, True):
+        print("case graphs", graphs, flush=True)
+        with Server(panel, precision, chunk_snps=32, graphs=graphs) as srv:
+            ids = [srv.submit(Y, XL, store_all=True, **THR) for Y, XL in reqs[:2]]
+            answers = [srv.wait(i) for i in ids]
+            answers.append(srv.wait(srv.submit(*reqs[2], store_all=True, **THR)))
+            replayed = srv.graph_chunks()
+        out[graphs] = answers
+        print("graphs", graphs, "replayed chunks", replayed, flush=True)
+        if graphs:
+            assert replayed >= 7          # the third request alone: 8 full chunks, the first eager
+        else:
+            assert replayed == 0
+    panel.close()
+    for a, b in zip(out[False], out[True]):
+        assert a.stats["n_hits"] == b.stats["n_hits"] > 0
+        if tol == 0.0:
+            for f in ("trait", "snp", "h", "n_used", "b", "se", "t", "p", "r2"):
+                np.testing.assert_array_equal(a.hits[f], b.hits[f])
+        else:
+            assert_hits_equal(a.hits, b.hits, tol)
