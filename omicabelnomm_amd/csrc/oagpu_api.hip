@@ -71,38 +71,68 @@ static bool chol_packed(std::vector<double>& A, int l) {
   return true;
 }
 
-// Smallest singular value of XL via Jacobi on XL'XL (the reference uses
-// sgesvd for the same test, src/Algorithm.cpp:508-522).
-static double min_singular(const std::vector<double>& S, int l) {
-  std::vector<double> A = S;
-  for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0.0;
-    for (int i = 0; i < l; ++i)
-      for (int j = i + 1; j < l; ++j) off += A[i * l + j] * A[i * l + j];
-    if (off < 1e-300) break;
+// Smallest singular value of XL (n x l, column stride ld), the reference's
+// linear-dependence test (LAPACKE_sgesvd, src/Algorithm.cpp:508-522).  A
+// Householder QR of XL in FP64 gives R with the same singular values; a
+// one-sided (Hestenes) Jacobi on R's columns then converges to them with
+// relative accuracy.  Working on XL'XL instead would square the condition
+// number: at n = 10,000 its eigenvalues near (1e-6)^2 are below the
+// rounding of the largest.
+static double min_singular(const double* XL, int n, int l, long long ld) {
+  std::vector<double> A((size_t)n * l);
+  for (int j = 0; j < l; ++j)
+    for (int k = 0; k < n; ++k) A[(size_t)j * n + k] = XL[(size_t)j * ld + k];
+  std::vector<double> R((size_t)l * l, 0.0);   // column-major, upper
+  for (int k = 0; k < l && k < n; ++k) {
+    double* a = &A[(size_t)k * n];
+    double nrm = 0.0;
+    for (int i = k; i < n; ++i) nrm = std::hypot(nrm, a[i]);
+    const double alpha = a[k] > 0 ? -nrm : nrm;
+    if (nrm > 0.0) {            // (a zero remainder needs no reflection)
+      a[k] -= alpha;            // v = x - alpha e_k, stored in place
+      const double vtv = -2.0 * alpha * a[k];   // |v|^2 = 2 nrm (nrm + |x_k|)
+      for (int j = k + 1; j < l; ++j) {
+        double* b = &A[(size_t)j * n];
+        double d = 0.0;
+        for (int i = k; i < n; ++i) d += a[i] * b[i];
+        const double f = 2.0 * d / vtv;
+        for (int i = k; i < n; ++i) b[i] -= f * a[i];
+      }
+    }
+    R[(size_t)k * l + k] = alpha;
+    for (int j = k + 1; j < l; ++j) R[(size_t)j * l + k] = A[(size_t)j * n + k];
+  }
+  // one-sided Jacobi: rotate column pairs of R until all are orthogonal
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    bool rotated = false;
     for (int p = 0; p < l; ++p)
       for (int q = p + 1; q < l; ++q) {
-        const double apq = A[p * l + q];
-        if (apq == 0.0) continue;
-        const double th = (A[q * l + q] - A[p * l + p]) / (2.0 * apq);
-        const double tt = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
-        const double c = 1.0 / std::sqrt(tt * tt + 1.0), s = tt * c;
-        for (int k = 0; k < l; ++k) {
-          const double akp = A[k * l + p], akq = A[k * l + q];
-          A[k * l + p] = c * akp - s * akq;
-          A[k * l + q] = s * akp + c * akq;
+        double* cp = &R[(size_t)p * l];
+        double* cq = &R[(size_t)q * l];
+        double a = 0.0, b = 0.0, g = 0.0;
+        for (int i = 0; i < l; ++i) {
+          a += cp[i] * cp[i];
+          b += cq[i] * cq[i];
+          g += cp[i] * cq[i];
         }
-        for (int k = 0; k < l; ++k) {
-          const double apk = A[p * l + k], aqk = A[q * l + k];
-          A[p * l + k] = c * apk - s * aqk;
-          A[q * l + k] = s * apk + c * aqk;
+        if (g == 0.0 || std::fabs(g) <= 1e-15 * std::sqrt(a * b)) continue;
+        rotated = true;
+        const double zeta = (b - a) / (2.0 * g);
+        const double t = (zeta >= 0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
+        const double cs = 1.0 / std::sqrt(1.0 + t * t), sn = cs * t;
+        for (int i = 0; i < l; ++i) {
+          const double x = cp[i], y = cq[i];
+          cp[i] = cs * x - sn * y;
+          cq[i] = sn * x + cs * y;
         }
       }
+    if (!rotated) break;
   }
   double mn = INFINITY;
-  for (int i = 0; i < l; ++i) {
-    const double ev = A[i * l + i];
-    mn = std::min(mn, ev > 0 ? std::sqrt(ev) : 0.0);
+  for (int j = 0; j < l; ++j) {
+    double nrm = 0.0;
+    for (int i = 0; i < l; ++i) nrm = std::hypot(nrm, R[(size_t)j * l + i]);
+    mn = std::min(mn, nrm);
   }
   return mn;
 }
@@ -301,7 +331,7 @@ extern "C" int oagpu_set_covariates(oagpu_ctx* c, const double* XL, long long ld
     for (int k = 0; k < n; ++k)
       if (std::isnan(XL[(size_t)j * ld + k]))
         return fail(c, OAGPU_ERR_ARG, "covariates contain NaN (rows must be excluded first)");
-  if (min_singular(S, l) < 0.000001)
+  if (min_singular(XL, n, l, ld) < 0.000001)
     return fail(c, OAGPU_ERR_COVARIATES,
                 "Error, the covariate matrix contains linearly dependent columns. Please review and "
                 "change it accordingly.");

@@ -37,6 +37,7 @@
  */
 #include "oracle.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -113,35 +114,25 @@ struct TraitData {
 }  // namespace
 
 extern "C" double oa_oracle_erfinv(double x) {
-    if (x <= -1.0) return -INFINITY;
+    /* Inverse of erf by bisection: erf(y) = x for |x| < 0.5, erfc(y) = 1 - x
+     * above (1 - x is exact there and erfc keeps the relative accuracy that
+     * erf loses near 1).  The bracket is halved until it no longer shrinks,
+     * i.e. to adjacent doubles. */
+    if (x < 0.0) return -oa_oracle_erfinv(-x);
     if (x >= 1.0) return INFINITY;
-    /* Giles (2010) single-precision starting point, then Newton steps on
-     * erf(y) - x using std::erf; converges to double precision. */
-    double w = -std::log((1.0 - x) * (1.0 + x));
-    double y;
-    if (w < 5.0) {
-        w -= 2.5;
-        double q = 2.81022636e-08;
-        q = 3.43273939e-07 + q * w; q = -3.5233877e-06 + q * w;
-        q = -4.39150654e-06 + q * w; q = 0.00021858087 + q * w;
-        q = -0.00125372503 + q * w; q = -0.00417768164 + q * w;
-        q = 0.246640727 + q * w; q = 1.50140941 + q * w;
-        y = q * x;
-    } else {
-        w = std::sqrt(w) - 3.0;
-        double q = -0.000200214257;
-        q = 0.000100950558 + q * w; q = 0.00134934322 + q * w;
-        q = -0.00367342844 + q * w; q = 0.00573950773 + q * w;
-        q = -0.0076224613 + q * w; q = 0.00943887047 + q * w;
-        q = 1.00167406 + q * w; q = 2.83297682 + q * w;
-        y = q * x;
+    if (x == 0.0) return 0.0;
+    const bool small = x < 0.5;
+    const double target = small ? x : 1.0 - x;
+    double lo = 0.0, hi = small ? 0.5 : 27.3;   /* erf(0.5) > 0.5, erfc(27.3) < 1e-300 */
+    for (int it = 0; it < 2000; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (mid <= lo || mid >= hi) break;
+        const bool below = small ? (std::erf(mid) < target) : (std::erfc(mid) > target);
+        if (below) lo = mid; else hi = mid;
     }
-    const double two_over_sqrtpi = 1.1283791670955126;
-    for (int it = 0; it < 4; ++it) {
-        double err = std::erf(y) - x;
-        y -= err / (two_over_sqrtpi * std::exp(-y * y));
-    }
-    return y;
+    /* the end of the bracket whose function value is closer */
+    const double flo = small ? std::erf(lo) : std::erfc(lo), fhi = small ? std::erf(hi) : std::erfc(hi);
+    return std::fabs(flo - target) <= std::fabs(fhi - target) ? lo : hi;
 }
 
 extern "C" double oa_oracle_tvalue(float pval) {
@@ -158,44 +149,42 @@ extern "C" double oa_oracle_pvalue(double t) {
 }
 
 extern "C" double oa_oracle_min_singular(const double *XL, int n, int l) {
-    /* Singular values of XL = sqrt(eigenvalues of XL'XL), Jacobi sweeps. */
-    std::vector<double> A(l * l, 0.0);
-    for (int i = 0; i < l; ++i)
-        for (int j = 0; j < l; ++j) {
-            double s = 0.0;
-            for (int k = 0; k < n; ++k) s += XL[i * (size_t)n + k] * XL[j * (size_t)n + k];
-            A[i * l + j] = s;
-        }
-    for (int sweep = 0; sweep < 100; ++sweep) {
-        double off = 0.0;
-        for (int i = 0; i < l; ++i)
-            for (int j = i + 1; j < l; ++j) off += A[i * l + j] * A[i * l + j];
-        if (off < 1e-300) break;
-        for (int pidx = 0; pidx < l; ++pidx)
-            for (int q = pidx + 1; q < l; ++q) {
-                double apq = A[pidx * l + q];
-                if (apq == 0.0) continue;
-                double theta = (A[q * l + q] - A[pidx * l + pidx]) / (2.0 * apq);
-                double tt = (theta >= 0 ? 1.0 : -1.0) /
-                            (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
-                double c = 1.0 / std::sqrt(tt * tt + 1.0), s = tt * c;
-                for (int k = 0; k < l; ++k) {
-                    double akp = A[k * l + pidx], akq = A[k * l + q];
-                    A[k * l + pidx] = c * akp - s * akq;
-                    A[k * l + q] = s * akp + c * akq;
+    /* Smallest singular value of XL (n x l), the reference's test
+     * (LAPACKE_sgesvd, src/Algorithm.cpp:508-522): one-sided (Hestenes)
+     * Jacobi on the columns of XL itself, rotating pairs until every pair
+     * is orthogonal; the singular values are then the column norms (no
+     * squaring of the condition number). */
+    std::vector<double> A(XL, XL + (size_t)n * l);
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        bool rotated = false;
+        for (int p = 0; p < l; ++p)
+            for (int q = p + 1; q < l; ++q) {
+                double *cp = &A[(size_t)p * n], *cq = &A[(size_t)q * n];
+                double a = 0.0, b = 0.0, g = 0.0;
+                for (int k = 0; k < n; ++k) {
+                    a += cp[k] * cp[k];
+                    b += cq[k] * cq[k];
+                    g += cp[k] * cq[k];
                 }
-                for (int k = 0; k < l; ++k) {
-                    double apk = A[pidx * l + k], aqk = A[q * l + k];
-                    A[pidx * l + k] = c * apk - s * aqk;
-                    A[q * l + k] = s * apk + c * aqk;
+                if (g == 0.0 || std::fabs(g) <= 1e-15 * std::sqrt(a * b)) continue;
+                rotated = true;
+                const double zeta = (b - a) / (2.0 * g);
+                const double t = (zeta >= 0 ? 1.0 : -1.0) /
+                                 (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
+                const double c = 1.0 / std::sqrt(1.0 + t * t), s = c * t;
+                for (int k = 0; k < n; ++k) {
+                    const double x = cp[k], y = cq[k];
+                    cp[k] = c * x - s * y;
+                    cq[k] = s * x + c * y;
                 }
             }
+        if (!rotated) break;
     }
     double mn = INFINITY;
-    for (int i = 0; i < l; ++i) {
-        double ev = A[i * l + i];
-        double sv = ev > 0 ? std::sqrt(ev) : 0.0;
-        if (sv < mn) mn = sv;
+    for (int j = 0; j < l; ++j) {
+        double s = 0.0;
+        for (int k = 0; k < n; ++k) s += A[(size_t)j * n + k] * A[(size_t)j * n + k];
+        mn = std::min(mn, std::sqrt(s));
     }
     return mn;
 }

@@ -209,3 +209,12 @@ def assert_mixed_records(hits, rec, label):
     print(label, "max|db|/SE %.2e max|d-log10p| %.2e strong hits %d same %s" % (db, dl, ns, same), flush=True)
     assert db <= MIXED_TOL and dl <= MIXED_TOL, (db, dl)
     assert same
+
+
+def near_singular(n, sv, seed):
+    """n x len(sv) matrix with the given singular values (random orthonormal
+    left and right factors)."""
+    rng = np.random.default_rng(seed)
+    U, _ = np.linalg.qr(rng.standard_normal((n, len(sv))))
+    V, _ = np.linalg.qr(rng.standard_normal((len(sv), len(sv))))
+    return np.asfortranarray(U @ np.diag(sv) @ V.T)

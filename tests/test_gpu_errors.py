@@ -12,7 +12,7 @@ import pytest
 
 from omicabelnomm_amd import OagpuError, Scanner, synth
 from omicabelnomm_amd import lib as L
-from parity_util import compare_fp64, run_oracle
+from parity_util import compare_fp64, near_singular, run_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -80,3 +80,29 @@ def test_hit_overflow_reports_the_needed_capacity(oracle):
     assert g.stats["n_hits"] == o["n_kept"] > 5
     assert "'n_hits': %d" % o["n_kept"] in str(e.value)
     compare_fp64(g, o)
+
+
+@pytest.mark.parametrize("smin", [0.8e-6, 1.25e-6])
+def test_rank_check_resolves_the_threshold(smin):
+    """The covariate rank check (the reference's sgesvd test: a singular
+    value below 1e-6 means linearly dependent columns) at n = 10,000, with
+    sigma_min 25 % either side of the threshold:
+    - sigma_max = 100: the rank check must refuse 0.8e-6 and pass 1.25e-6,
+      which the Gram matrix's FP64 Cholesky then refuses on its own
+      (condition number 8e7, squared 6e15); an eigenvalue test on XL'XL
+      could not tell the two apart (eigenvalues near 1e-12 under the
+      rounding of 1e4), so the messages show which test refused;
+    - sigma_max = 3: 1.25e-6 is accepted and the context scans."""
+    XL = near_singular(10_000, [100.0, 30.0, 5.0, 1.0, smin], seed=9)
+    print("case rank check", smin, flush=True)
+    with Scanner(10_000, 5, 1, "fp64", max_traits=4) as sc:
+        with pytest.raises(OagpuError) as e:
+            sc.set_covariates(XL)
+        assert e.value.code == L.ERR_COVARIATES
+        assert ("linearly dependent" in str(e.value)) == (smin < 1e-6), str(e.value)
+    XL = near_singular(10_000, [3.0, 1.0, 0.3, 0.05, smin], seed=10)
+    with Scanner(10_000, 5, 1, "fp64", max_traits=4) as sc:
+        if smin < 1e-6:
+            assert _code(sc.set_covariates, XL) == L.ERR_COVARIATES
+        else:
+            sc.set_covariates(XL)

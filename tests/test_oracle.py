@@ -10,6 +10,7 @@ import pytest
 import scipy.special as sp
 
 from omicabelnomm_amd import synth
+from parity_util import near_singular
 
 
 def _data(n=300, l=4, m=8, t=6, r=1, miss_rate=0.0, seed=0):
@@ -148,7 +149,11 @@ def test_pvalue_rule(oracle):
 def test_tvalue_and_erfinv(oracle, p):
     want = np.float32(np.sqrt(2.0) * sp.erfinv(1.0 - np.float64(np.float32(p))))
     assert oracle.tvalue(p) == float(want)
-    np.testing.assert_allclose(oracle.erfinv(0.3), sp.erfinv(0.3), rtol=1e-14)
+
+
+@pytest.mark.parametrize("x", [0.3, -0.7, 1e-12, 0.49999, 0.5, 0.9, 1.0 - 1e-10, 1.0 - 2.0 ** -52])
+def test_erfinv_against_scipy(oracle, x):
+    np.testing.assert_allclose(oracle.erfinv(x), sp.erfinv(x), rtol=2e-15)
 
 
 def test_min_singular(oracle):
@@ -157,6 +162,21 @@ def test_min_singular(oracle):
     np.testing.assert_allclose(oracle.min_singular(A), np.linalg.svd(A, compute_uv=False).min(), rtol=1e-10)
     A[:, 3] = A[:, 2]
     assert oracle.min_singular(A) < 1e-6
+
+
+@pytest.mark.parametrize("smin", [0.8e-6, 1.25e-6, 1e-9])
+def test_min_singular_near_the_threshold(oracle, smin):
+    """At n = 10,000 with sigma_max = 100, sigma_min near the reference's
+    1e-6 threshold: Jacobi on XL'XL could not resolve it (its eigenvalues
+    near 1e-12 sit below the rounding of 1e4); the one-sided Jacobi on XL
+    agrees with numpy's SVD to 1e-6 relative and puts the matrix on the
+    right side of the threshold."""
+    A = near_singular(10_000, [100.0, 30.0, 5.0, 1.0, smin], seed=9)
+    want = np.linalg.svd(A, compute_uv=False).min()
+    got = oracle.min_singular(A)
+    np.testing.assert_allclose(got, want, rtol=1e-6)
+    np.testing.assert_allclose(got, smin, rtol=1e-6)
+    assert (got < 1e-6) == (smin < 1e-6)
 
 
 def test_exclusion_list_example(oracle):
