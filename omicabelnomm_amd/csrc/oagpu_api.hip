@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -210,7 +211,10 @@ extern "C" int oagpu_create(const oagpu_params* prm, oagpu_ctx** out) {
   c->c_size = prm->precision == OAGPU_FP64 ? 8 : 4;
   c->ldop = (long long)c->n_pad * (prm->precision == OAGPU_BF16X3 ? 2 : 1);
   c->hit_cap = prm->max_hits > 0 ? prm->max_hits : (1ll << 22);
+  // fallback entries per chunk; pairs beyond them compute the explicit
+  // residual inside the pair kernel (OAGPU_FB_CAP sets the capacity)
   c->fb_cap = 1 << 16;
+  if (const char* e = getenv("OAGPU_FB_CAP")) c->fb_cap = std::max(1, atoi(e));
   {
     std::vector<float> ones((size_t)c->n_pad, 1.0f);
     HIPCK(c, hipMalloc(&c->E32, sizeof(float) * ones.size()));

@@ -323,3 +323,31 @@ def test_tiny_shapes(name, precision, oracle_mod):
         db, dl = precision_report(g, o)
         assert db < 1e-3 and dl < 1e-3, (db, dl)
         np.testing.assert_array_equal(g.dense["n_used"], o["n_used"])
+
+
+@pytest.mark.parametrize("miss", ["mcar", "groups"])
+def test_fallback_arena_overflow(miss, oracle_mod, monkeypatch):
+    """Pairs beyond the fallback arena (a trait with observed zeros sends
+    every one of its pairs there) compute the explicit residual inside the
+    pair kernel: with an arena of 5 entries per chunk, both pair kernels
+    (many patterns, and the shared-pattern table) still match the oracle,
+    dense and through the hit list."""
+    monkeypatch.setenv("OAGPU_FB_CAP", "5")
+    # groups: 2 patterns of 40 traits, so the table kernel's waves are
+    # pattern-pure (scalar table rows)
+    data = synth.make(n=320, l=3, m=40, t=24 if miss == "mcar" else 80, miss=miss, miss_rate=0.05,
+                      pattern_groups=2, pattern_rate=0.05, planted=6, seed=7)
+    Y = data.Y.clone()
+    Y[0, :25] = 0.0
+    Y[3, :7] = 0.0
+    data.Y = Y
+    print("case fallback overflow", miss, flush=True)
+    g = run_gpu(data, "fp64", store_all=True)
+    o = run_oracle(oracle_mod, data, store_all=True)
+    compare_fp64(g, o, rtol=1e-9)
+    assert g.stats["n_fallback"] >= 2 * data.m > 5
+    thr = dict(min_p_store=0.05, min_p_disp=0.05, min_r2_store=-10.0)
+    gs = run_gpu(data, "fp64", dense=False, **thr)
+    want = hits_from_dense(run_oracle(oracle_mod, data, **thr), data.m)
+    got = np.stack([gs.hits["trait"], gs.hits["snp"], gs.hits["h"]], axis=1)
+    np.testing.assert_array_equal(got, want)
