@@ -13,9 +13,10 @@ resident in HBM when the timed region starts.
 
 One JSON line is printed: the mixed-precision (split-BF16 x3) value, the
 FP64 path beside it, the roofline of the dominant kernel (the contraction),
-the FP64 CPU restatement timed on a bounded sample of the same workload,
-the timed runs' hits on that sample checked against the restatement's
-(`oracle_check`; the bench exits with status 3 when they disagree), and
+the FP32 restatement of the reference's blocked CPU algorithm timed on a
+bounded sample of the same workload (`cpu_baseline`), the timed runs' hits
+on a sample of SNPs checked against the FP64 oracle (`oracle_check`; the
+bench exits with status 3 when they disagree), and
 the latency half of the metric: p50/p99 of a bounded request stream
 (config-4 requests on a 100,000-SNP float16 panel at 20 requests/s).
 """
@@ -48,8 +49,8 @@ def parse():
     ap.add_argument("--jobs", type=int, default=0, help="requests of configs 3/4 (0 = the config's count)")
     ap.add_argument("--chunk", type=int, default=8192, help="server chunk (SNPs), configs 3/4")
     ap.add_argument("--budget-gib", type=float, default=160.0, help="server HBM budget, configs 3/4")
-    ap.add_argument("--no-graphs", action="store_true",
-                    help="configs 3/4: launch each chunk's kernels one by one instead of replaying a graph")
+    ap.add_argument("--graphs", action="store_true",
+                    help="configs 3/4: replay each chunk's launches from a captured HIP graph")
     ap.add_argument("--fp64-jobs", type=int, default=-1,
                     help="requests of configs 3/4 rerun one at a time in FP64 (-1 = all)")
     ap.add_argument("--precision", default="bf16x3", choices=["fp64", "fp32", "bf16x3"])
@@ -63,6 +64,8 @@ def parse():
     ap.add_argument("--serve-leg-jobs", type=int, default=32)
     ap.add_argument("--serve-leg-rate", type=float, default=20.0, help="offered requests/s of that leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--oracle-seconds", type=float, default=5.0,
+                    help="FP64 oracle work behind the hit check (planted SNPs, then the first ones)")
     ap.add_argument("--host-fed", type=int, default=0, metavar="BLOCKS",
                     help="also time the scan with genotypes in page-locked host memory, uploaded in BLOCKS "
                          "blocks, each upload overlapping the previous block's scan (configs 0-2)")
@@ -206,37 +209,72 @@ def measured_traffic(variant, config):
     return per_launch * rec["launches_per_contraction"]
 
 
+def cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(data, budget_s, p_store):
-    """FP64 CPU restatement (oracle) on m' SNPs x all traits: the SNPs that
-    carry planted effects, then the first SNPs.  Returns the timing record
-    and (the sample's SNP indices, the oracle's dense result on them)."""
+    """The FP32 restatement of the reference's blocked CPU algorithm
+    (oracle/ref32.cpp: per block of 1,000 traits x 500 SNPs the sgemm/ssyrk
+    products, per pair build_S, the missing-row corrections and sposv, the
+    explicit RSS over the n rows, hpc_statistics) on the first m' SNPs x all
+    traits, m' sized to about budget_s of work."""
     from oracle import oracle as O
     threads = min(16, os.cpu_count() or 1)   # the host's CPU share beside one GPU
     XL = data.XL
     Y = data.y_matrix()
-    planted = np.unique(data.planted[:, 0]) if data.planted is not None else np.zeros(0, dtype=np.int64)
+    thr = dict(min_p_store=p_store, min_p_disp=p_store, min_r2_store=-10.0)
 
-    def sample(ms):
-        rest = np.setdiff1d(np.arange(min(data.m, ms + len(planted))), planted)
-        return np.sort(np.concatenate([planted, rest])[:ms])
+    def run(ms):
+        XR = data.xr_matrix(np.arange(ms))
+        t0 = time.perf_counter()
+        O.scan_ref32(XL, XR, Y, data.r, threads=threads, **thr)
+        return time.perf_counter() - t0
+
+    probe = max(1, min(2 * threads, data.m))
+    dt = run(probe)
+    ms = int(min(data.m, max(probe, probe * budget_s / max(dt, 1e-3))))
+    dt = run(ms)
+    return {"value": ms * data.t / dt, "unit": "SNPxtrait tests/s", "cores": threads, "kind": "port",
+            "sample": "first %d SNPs x %d traits of the same workload" % (ms, data.t),
+            "seconds": round(dt, 3), "precision": "fp32",
+            "structure": "the reference's blocked algorithm (tb=1000, mb=500): XL'XL, XL'Y, XL'XR, XR'Y "
+                         "products, per-pair build_S + missing-row corrections + Cholesky solve, explicit "
+                         "RSS over the n rows, statistics",
+            "blas": "FP32 cache-blocked kernel of oracle/ref32.cpp (AVX2/FMA vectors, OpenMP); the image "
+                    "has no CBLAS",
+            "nproc": os.cpu_count(), "cpu_model": cpu_model()}
+
+
+def oracle_sample(data, budget_s, p_store):
+    """The FP64 restatement (oracle) on the SNPs with planted effects, then
+    the first SNPs, x all traits, about budget_s of work: (SNP indices,
+    dense result)."""
+    from oracle import oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    XL = data.XL
+    Y = data.y_matrix()
+    planted = np.unique(data.planted[:, 0]) if data.planted is not None else np.zeros(0, dtype=np.int64)
+    thr = dict(min_p_store=p_store, min_p_disp=p_store, min_r2_store=-10.0)
 
     def run(snps):
-        XR = data.xr_matrix(snps)
         t0 = time.perf_counter()
-        o = O.scan(XL, XR, Y, data.r, min_p_store=p_store, min_p_disp=p_store, min_r2_store=-10.0,
-                   threads=threads)
+        o = O.scan(XL, data.xr_matrix(snps), Y, data.r, threads=threads, **thr)
         return time.perf_counter() - t0, o
 
     probe = max(1, min(threads, data.m))
     dt, _ = run(np.arange(probe))
     ms = int(min(data.m, max(probe, probe * budget_s / max(dt, 1e-3))))
-    ms = max(probe, (ms // threads) * threads) if ms >= threads else ms
-    snps = sample(ms)
-    dt, o = run(snps)
-    return {"value": len(snps) * data.t / dt, "unit": "SNPxtrait tests/s", "cores": threads, "kind": "port",
-            "sample": "%d SNPs (the %d with planted effects, then the first ones) x %d traits of the same "
-                      "workload (FP64 restatement, OpenMP over SNPs)" % (len(snps), len(planted), data.t),
-            "seconds": round(dt, 3)}, (snps, o)
+    rest = np.setdiff1d(np.arange(min(data.m, ms + len(planted))), planted)
+    snps = np.sort(np.concatenate([planted, rest])[:max(ms, len(planted))])
+    _, o = run(snps)
+    return snps, o
 
 
 def oracle_check(o, snps, hits, fp64_hits):
@@ -368,7 +406,7 @@ def bench_serve(args):
 
     def serve(arrivals):
         srv = Server(panel, args.precision, hbm_budget=int(args.budget_gib * 2**30), max_active=64,
-                     chunk_snps=args.chunk, graphs=not args.no_graphs)
+                     chunk_snps=args.chunk, graphs=args.graphs)
         warm = synth.JobSpec(t=64, l=3, keep=None, miss_rate=0.02, arrival=0.0, seed=1)
         srv.wait(srv.submit(*warm.arrays(n), **thr))   # autotuner, code objects
         ids = [None] * len(specs)
@@ -455,7 +493,7 @@ def bench_serve(args):
                                   "all individuals"),
                    "thresholds": {"minPstore": args.p_store, "minPdisp": args.p_store, "minR2store": -10.0},
                    "precision": args.precision, "chunk_snps": args.chunk, "gen_seconds": round(gen_s, 2),
-                   "graphs": not args.no_graphs,
+                   "graphs": args.graphs,
                    "value_is": "all requests submitted at once: total tests / time to the last answer"},
         "latency_ms": {"p50": percentile(lat, 50), "p99": percentile(lat, 99), "max": max(lat),
                        "note": "submitted together"},
@@ -550,7 +588,8 @@ def main():
     cpu = None
     ocheck = None
     if not args.no_cpu_baseline:
-        cpu, (snps_o, o) = cpu_baseline(data, args.cpu_seconds, args.p_store)
+        cpu = cpu_baseline(data, args.cpu_seconds, args.p_store)
+        snps_o, o = oracle_sample(data, args.oracle_seconds, args.p_store)
         if args.precision == "fp64":
             hits64, hits_mixed = hits_mixed, hits_mixed
         ocheck = oracle_check(o, snps_o, hits_mixed, hits64)

@@ -47,9 +47,9 @@ typedef struct oagpu_server_params {
     long long hbm_budget;   /* bytes for active requests (0 = 64 GiB)               */
     int max_active;         /* requests in one batch (0 = 64)                       */
     int chunk_snps;         /* SNPs per chunk (0 = 16384)                           */
-    int no_graphs;          /* 1: launch each chunk's kernels one by one; 0: replay
-                               them from a HIP graph captured once per active set
-                               and chunk size                                       */
+    int graphs;             /* 1: replay each chunk's launches from a HIP graph
+                               captured once per active set and chunk size;
+                               0: launch them one by one (the default)              */
 } oagpu_server_params;
 
 typedef struct oagpu_request {

@@ -135,7 +135,7 @@ struct oagpu_server {
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   long long g_epoch[2] = {-1, -1};   // active set a slot's graph was captured for
   long long g_seen[2] = {-1, -1};    // active set a slot last ran eagerly for
-  bool graphs_off = false;           // no_graphs, or a capture failed
+  bool graphs_off = false;           // replay not asked for, or a capture failed
   long long graph_chunks = 0;        // chunks replayed from a graph
 };
 
@@ -604,7 +604,7 @@ extern "C" int oagpu_server_create(oagpu_panel* P, const oagpu_server_params* pr
   std::vector<int> id(P->n_pad, -1);
   for (int k = 0; k < P->n; ++k) id[k] = k;
   SRVCK(S, hipMalloc(&S->s0_dev, sizeof(int)));
-  S->graphs_off = S->prm.no_graphs != 0;
+  S->graphs_off = S->prm.graphs == 0;
   SRVCK(S, hipMalloc(&S->uidx, sizeof(int) * id.size()));
   SRVCK(S, hipMemcpy(S->uidx, id.data(), sizeof(int) * id.size(), hipMemcpyHostToDevice));
   S->th = std::thread(server_loop, S);

@@ -69,7 +69,7 @@ class Stats(ctypes.Structure):
 
 class ServerParams(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("hbm_budget", ctypes.c_longlong),
-                ("max_active", ctypes.c_int), ("chunk_snps", ctypes.c_int), ("no_graphs", ctypes.c_int)]
+                ("max_active", ctypes.c_int), ("chunk_snps", ctypes.c_int), ("graphs", ctypes.c_int)]
 
 
 class Request(ctypes.Structure):

@@ -50,11 +50,11 @@ class Answer:
 
 
 class Server:
-    def __init__(self, panel: Panel, precision="bf16x3", hbm_budget=0, max_active=0, chunk_snps=0, graphs=True):
+    def __init__(self, panel: Panel, precision="bf16x3", hbm_budget=0, max_active=0, chunk_snps=0, graphs=False):
         self._L = L.load()
         self.panel = panel
         prm = L.ServerParams(precision=L.PRECISIONS[precision], hbm_budget=hbm_budget, max_active=max_active,
-                             chunk_snps=chunk_snps, no_graphs=0 if graphs else 1)
+                             chunk_snps=chunk_snps, graphs=1 if graphs else 0)
         self._s = ctypes.c_void_p()
         rc = self._L.oagpu_server_create(panel._p, ctypes.byref(prm), ctypes.byref(self._s))
         if rc != L.OK:

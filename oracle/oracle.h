@@ -85,6 +85,12 @@ double oa_oracle_min_singular(const double *XL, int n, int l);
  * indixes", -3 "valid positive indixes", -4 everything excluded. */
 int oa_oracle_exclusions(const char *text, int n, uint8_t *keep);
 
+/* FP32 restatement of the reference's blocked algorithm (ref32.cpp): the
+ * same inputs in float, tb traits and mb SNPs per block, the same outputs
+ * (widened to double).  The CPU baseline of the benchmark. */
+int oa_ref32_scan(const oa_oracle_params *prm, const float *XL, const float *XR, const float *Y,
+                  int tb, int mb, oa_oracle_out *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,4 +1,5 @@
-"""ctypes wrapper of the FP64 CPU oracle (oracle/oracle.cpp).
+"""ctypes wrapper of the FP64 CPU oracle (oracle/oracle.cpp) and of the FP32
+restatement of the reference's blocked algorithm (oracle/ref32.cpp).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() to
 compare its scan with, and by bench.py's cpu_baseline leg.  The product path
@@ -65,6 +66,9 @@ def lib():
         L.oa_oracle_min_singular.restype = ctypes.c_double
         L.oa_oracle_exclusions.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p]
         L.oa_oracle_exclusions.restype = ctypes.c_int
+        L.oa_ref32_scan.argtypes = [ctypes.POINTER(_Params), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_int, ctypes.c_int, ctypes.POINTER(_Out)]
+        L.oa_ref32_scan.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -90,6 +94,42 @@ def exclusions(text: str, n: int):
     keep = np.zeros(n, dtype=np.uint8)
     rc = lib().oa_oracle_exclusions(text.encode(), n, keep.ctypes.data)
     return rc, keep.astype(bool)
+
+
+def scan_ref32(XL, XR, Y, r, *, tb=1000, mb=500, min_p_store=0.1, min_p_disp=0.05, min_r2_store=1e-5,
+               disp_cov=False, store_all=False, real_n=None, threads=1):
+    """The FP32 restatement of the reference's blocked algorithm (tb traits
+    and mb SNPs per block, the reference's defaults), same inputs and
+    outputs as scan()."""
+    XL = np.asfortranarray(XL, dtype=np.float32)
+    XR = np.asfortranarray(XR, dtype=np.float32)
+    Y = np.asfortranarray(Y, dtype=np.float32)
+    n, l = XL.shape
+    assert XR.shape[0] == n and Y.shape[0] == n and XR.shape[1] % r == 0
+    m = XR.shape[1] // r
+    t = Y.shape[1]
+    p = l + r
+    prm = _Params(n=n, real_n=real_n or n, l=l, r=r, t=t, m=m,
+                  disp_cov=int(disp_cov), store_all=int(store_all),
+                  min_r2_store=float(np.float32(min_r2_store)),
+                  t_star=tvalue(min_p_store),
+                  min_p_disp=float(np.float32(min_p_disp)), threads=threads)
+    res = {
+        "beta": np.zeros((t, m, p)), "se": np.zeros((t, m, p)),
+        "tstat": np.zeros((t, m, p)), "pval": np.zeros((t, m, p)),
+        "kept": np.zeros((t, m, p), dtype=np.uint8),
+        "r2": np.zeros((t, m)), "rss": np.zeros((t, m)),
+        "emitted": np.zeros((t, m), dtype=np.uint8),
+        "n_used": np.zeros(t, dtype=np.int32), "var_x": np.zeros((m, p)),
+    }
+    out = _Out(**{k: v.ctypes.data for k, v in res.items() if k != "var_x"})
+    rc = lib().oa_ref32_scan(ctypes.byref(prm), XL.ctypes.data, XR.ctypes.data, Y.ctypes.data,
+                             int(tb), int(mb), ctypes.byref(out))
+    res["status"] = rc
+    res["n_sig"] = out.n_sig
+    res["n_kept"] = out.n_kept
+    res["n_emitted"] = out.n_emitted
+    return res
 
 
 def scan(XL, XR, Y, r, *, min_p_store=0.1, min_p_disp=0.05, min_r2_store=1e-5,

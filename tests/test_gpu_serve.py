@@ -196,6 +196,8 @@ def test_subset_refused_on_a_panel_with_missing_genotypes():
     panel.close()
 
 
+@pytest.mark.skip(reason="graph replay is switched off: the default benchmark run ended in a host "
+                         "segmentation fault with replay on in its request-stream leg, cause not found")
 @pytest.mark.parametrize("precision,tol", [("fp64", 0.0), ("bf16x3", 1e-4)])
 def test_graph_replay_equals_launches(precision, tol):
     """A chunk's launches replayed from a captured HIP graph (the chunk start
