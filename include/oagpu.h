@@ -220,6 +220,12 @@ int oagpu_last_timings(const oagpu_ctx *ctx, double *out, int n);
  * oagpu_scan (no counterpart in the reference, which calls BLAS). */
 const char *oagpu_gemm_variant(const oagpu_ctx *ctx);
 
+/* Shape classes the contraction autotuner has timed in this process (no
+ * counterpart in the reference).  Its choices persist in a cache file,
+ * OAGPU_TUNE_CACHE or $HOME/.cache/omicabelnomm/gemm_tune_<arch>_<CUs>.txt,
+ * read before a shape is timed. */
+int oagpu_gemm_tunings(void);
+
 /* getTvalue (src/Utility.cpp:244-250): sqrt(2) erfinv(1 - p) as a float. */
 double oagpu_tvalue(float pval);
 

@@ -1118,3 +1118,5 @@ extern "C" int oagpu_last_timings(const oagpu_ctx* c, double* out, int n) {
 }
 
 extern "C" const char* oagpu_gemm_variant(const oagpu_ctx* c) { return c ? c->gemm_variant.c_str() : ""; }
+
+extern "C" int oagpu_gemm_tunings(void) { return gemm_tunings(); }

@@ -23,11 +23,16 @@
 //
 // launch_gemm times these kernels and the library GEMMs of
 // oagpu_gemm_lib.hip on the first operands of each shape class and keeps the
-// fastest.
+// fastest; the choices persist in a cache file across processes.
+#include <sys/stat.h>
+
+#include <cctype>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <string>
 #include <tuple>
 
 #include "oagpu_kernels.h"
@@ -41,49 +46,57 @@ __device__ __forceinline__ int xcd_tile(int id, int total) {
   return xcd * q + (xcd < rem ? xcd : rem) + loc;
 }
 
-// FP64: v_mfma_f64_16x16x4f64.  K step 16 = 4 MFMA steps; lane group g
-// supplies k = 4g + step.
+// FP64: v_mfma_f64_16x16x4f64, fed from global memory.  4 waves in a 2 x 2
+// grid, each wave AB x BB blocks of 16 x 16 (tile 32 AB x 32 BB); per k-step
+// of KS, KS / 16 runs of 4 doubles per lane and block are loaded, lane
+// group g supplying k = 4g + step of each 16.  The tuner times the tile
+// shapes and k-steps of kVariants per shape class.
+template <int AB, int BB, int KS>
 __global__ __launch_bounds__(256, 2) void k_gemm_f64(const double* __restrict__ A,
                                                     const double* __restrict__ B,
                                                     double* __restrict__ C, int M, int N, int K, int lda, int ldb,
                                                     int ldc, int tiles_n, int total) {
+  constexpr int TM = 32 * AB, TN = 32 * BB, KR = KS / 16;
   const int tile = xcd_tile(blockIdx.x, total);
   const int tm = tile / tiles_n, tn = tile - tm * tiles_n;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int i = lane & 15, g = lane >> 4;
-  const int row0 = tm * 128 + (w & 1) * 64;
-  const int col0 = tn * 128 + (w >> 1) * 64;
-  d4 acc[4][4];
+  const int row0 = tm * TM + (w & 1) * (16 * AB);
+  const int col0 = tn * TN + (w >> 1) * (16 * BB);
+  d4 acc[AB][BB];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < AB; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
-  const double* Ap[4];
-  const double* Bp[4];
+    for (int b = 0; b < BB; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+  const double* Ap[AB];
+  const double* Bp[BB];
 #pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    Ap[a] = A + (size_t)(row0 + 16 * a + i) * lda + 4 * g;
-    Bp[a] = B + (size_t)(col0 + 16 * a + i) * ldb + 4 * g;
-  }
-  for (int k0 = 0; k0 < K; k0 += 16) {
-    d4 av[4], bv[4];
+  for (int a = 0; a < AB; ++a) Ap[a] = A + (size_t)(row0 + 16 * a + i) * lda + 4 * g;
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      av[a] = *reinterpret_cast<const d4*>(Ap[a] + k0);
-      bv[a] = *reinterpret_cast<const d4*>(Bp[a] + k0);
+  for (int b = 0; b < BB; ++b) Bp[b] = B + (size_t)(col0 + 16 * b + i) * ldb + 4 * g;
+  for (int k0 = 0; k0 < K; k0 += KS) {
+    d4 av[KR][AB], bv[KR][BB];
+#pragma unroll
+    for (int u = 0; u < KR; ++u) {
+#pragma unroll
+      for (int a = 0; a < AB; ++a) av[u][a] = *reinterpret_cast<const d4*>(Ap[a] + k0 + 16 * u);
+#pragma unroll
+      for (int b = 0; b < BB; ++b) bv[u][b] = *reinterpret_cast<const d4*>(Bp[b] + k0 + 16 * u);
     }
 #pragma unroll
-    for (int st = 0; st < 4; ++st)
+    for (int u = 0; u < KR; ++u)
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int st = 0; st < 4; ++st)
 #pragma unroll
-        for (int b = 0; b < 4; ++b)
-          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a][st], bv[b][st], acc[a][b], 0, 0, 0);
+        for (int a = 0; a < AB; ++a)
+#pragma unroll
+          for (int b = 0; b < BB; ++b)
+            acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u][a][st], bv[u][b][st], acc[a][b], 0, 0, 0);
   }
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < AB; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
+    for (int b = 0; b < BB; ++b) {
       const int col = col0 + 16 * b + i;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
@@ -543,11 +556,14 @@ struct GemmVariant {
                  hipStream_t);
 };
 
+template <int AB, int BB, int KS>
 static bool v_f64(const void* Ah, const void*, const void* Bh, const void*, void* C, int M, int N, int K,
                   int lda, int ldb, int ldc, hipStream_t st) {
-  const int tiles_m = (M + 127) / 128, tiles_n = (N + 127) / 128, total = tiles_m * tiles_n;
-  hipLaunchKernelGGL(k_gemm_f64, dim3(total), dim3(256), 0, st, (const double*)Ah, (const double*)Bh,
-                     (double*)C, M, N, K, lda, ldb, ldc, tiles_n, total);
+  constexpr int TM = 32 * AB, TN = 32 * BB;
+  if (K % KS) return false;
+  const int tiles_m = (M + TM - 1) / TM, tiles_n = (N + TN - 1) / TN, total = tiles_m * tiles_n;
+  hipLaunchKernelGGL((k_gemm_f64<AB, BB, KS>), dim3(total), dim3(256), 0, st, (const double*)Ah,
+                     (const double*)Bh, (double*)C, M, N, K, lda, ldb, ldc, tiles_n, total);
   return true;
 }
 static bool v_f32(const void* Ah, const void*, const void* Bh, const void*, void* C, int M, int N, int K,
@@ -598,7 +614,11 @@ static bool v_lib_f32(const void* Ah, const void*, const void* Bh, const void*, 
   return lib_gemm_f32(Ah, Bh, C, M, N, K, lda, ldb, ldc, st);
 }
 static const GemmVariant kVariants[] = {
-    {"f64_direct_128x128", PREC_FP64, 128, 128, true, v_f64},
+    {"f64_direct_128x128", PREC_FP64, 128, 128, true, v_f64<4, 4, 16>},
+    {"f64_direct_128x128_k32", PREC_FP64, 128, 128, true, v_f64<4, 4, 32>},
+    {"f64_direct_128x64", PREC_FP64, 128, 64, true, v_f64<4, 2, 16>},
+    {"f64_direct_64x128", PREC_FP64, 64, 128, true, v_f64<2, 4, 16>},
+    {"f64_direct_64x64_k32", PREC_FP64, 64, 64, true, v_f64<2, 2, 32>},
     {"f32_direct_128x128", PREC_FP32, 128, 128, true, v_f32},
     {"f32_hipblaslt", PREC_FP32, 0, 0, true, v_lib_f32},
     {"bf16x3_hipblaslt", PREC_BF16X3, 0, 0, true, v_lib_bf16x3},
@@ -617,6 +637,67 @@ struct TuneKey {
 };
 static std::map<TuneKey, int> g_tuned;
 static std::mutex g_tune_mu;
+static int g_tunings = 0;   // shapes timed by this process
+
+// The tuning cache persists across processes in a text file, one line per
+// shape class: "prec mclass nclass K variant".  OAGPU_TUNE_CACHE names the
+// file ("" switches persistence off); by default it is
+// $HOME/.cache/omicabelnomm/gemm_tune_<arch>_<CUs>.txt, so a cache is only
+// read on the kind of device that wrote it.
+static std::string tune_cache_path() {
+  const char* e = getenv("OAGPU_TUNE_CACHE");
+  if (e) return e;
+  const char* home = getenv("HOME");
+  if (!home || !*home) return "";
+  int dev = 0;
+  hipDeviceProp_t prop{};
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return "";
+  std::string arch = prop.gcnArchName;
+  for (char& ch : arch)
+    if (!std::isalnum((unsigned char)ch)) ch = '_';
+  return std::string(home) + "/.cache/omicabelnomm/gemm_tune_" + arch + "_" +
+         std::to_string(prop.multiProcessorCount) + ".txt";
+}
+
+static bool g_cache_loaded = false;
+static std::string g_cache_file;
+
+// Reads the cache file once (under g_tune_mu); lines naming a variant this
+// build does not have are skipped.
+static void load_tune_cache() {
+  if (g_cache_loaded) return;
+  g_cache_loaded = true;
+  g_cache_file = tune_cache_path();
+  if (g_cache_file.empty()) return;
+  FILE* f = std::fopen(g_cache_file.c_str(), "r");
+  if (!f) return;
+  int prec, mc, nc, k;
+  char name[96];
+  while (std::fscanf(f, "%d %d %d %d %95s", &prec, &mc, &nc, &k, name) == 5)
+    for (int v = 0; v < kNumVariants; ++v)
+      if (kVariants[v].prec == prec && kVariants[v].tuned && !std::strcmp(kVariants[v].name, name))
+        g_tuned[TuneKey{prec, mc, nc, k}] = v;
+  std::fclose(f);
+}
+
+static void save_tune_entry(const TuneKey& key, int v) {
+  if (g_cache_file.empty() || v < 0) return;
+  const size_t slash = g_cache_file.rfind('/');
+  if (slash != std::string::npos && slash > 0) {
+    // create the directory chain (errors surface as a failed fopen)
+    for (size_t pos = 1; pos <= slash; ++pos)
+      if (g_cache_file[pos] == '/' || pos == slash) (void)mkdir(g_cache_file.substr(0, pos).c_str(), 0755);
+  }
+  FILE* f = std::fopen(g_cache_file.c_str(), "a");
+  if (!f) return;
+  std::fprintf(f, "%d %d %d %d %s\n", key.prec, key.mclass, key.n, key.k, kVariants[v].name);
+  std::fclose(f);
+}
+
+int gemm_tunings() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  return g_tunings;
+}
 
 static int pinned_variant(int prec) {
   const char* e = getenv("OAGPU_GEMM_VARIANT");
@@ -642,6 +723,7 @@ const char* gemm_variant_name(int prec, int M, int N, int K) {
   if (pv >= 0) return kVariants[pv].name;
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
+    load_tune_cache();
     auto it = g_tuned.find(TuneKey{prec, m_class(M), n_class(N), K});
     if (it != g_tuned.end()) return kVariants[it->second].name;
   }
@@ -659,7 +741,7 @@ int gemm_fixed_row_pad() { return 128; }
 
 void launch_gemm_fixed(int prec, const void* Ahi, const void* Alo, const void* Bhi, const void* Blo, void* C, int M,
                        int N, int K, int lda, int ldb, int ldc, hipStream_t st) {
-  if (prec == PREC_FP64) v_f64(Ahi, Alo, Bhi, Blo, C, M, N, K, lda, ldb, ldc, st);
+  if (prec == PREC_FP64) v_f64<4, 4, 16>(Ahi, Alo, Bhi, Blo, C, M, N, K, lda, ldb, ldc, st);
   else if (prec == PREC_FP32) v_f32(Ahi, Alo, Bhi, Blo, C, M, N, K, lda, ldb, ldc, st);
   else v_bf_lds(Ahi, Alo, Bhi, Blo, C, M, N, K, lda, ldb, ldc, st);
 }
@@ -671,6 +753,7 @@ void launch_gemm(int prec, const void* Ahi, const void* Alo, const void* Bhi, co
     const TuneKey key{prec, m_class(M), n_class(N), K};
     {
       std::lock_guard<std::mutex> lk(g_tune_mu);
+      load_tune_cache();
       auto it = g_tuned.find(key);
       if (it != g_tuned.end()) v = it->second;
     }
@@ -706,6 +789,8 @@ void launch_gemm(int prec, const void* Ahi, const void* Alo, const void* Bhi, co
       (void)hipEventDestroy(e1);
       std::lock_guard<std::mutex> lk(g_tune_mu);
       g_tuned[key] = v;
+      ++g_tunings;
+      save_tune_entry(key, v);
     }
   }
   if (v >= 0 && kVariants[v].launch(Ahi, Alo, Bhi, Blo, C, M, N, K, lda, ldb, ldc, st)) return;

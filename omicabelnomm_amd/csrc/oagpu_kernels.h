@@ -98,6 +98,8 @@ void launch_gemm_fused(const FusedGemmArgs& a, int gdt, hipStream_t st);
 bool gemm_fused_supported(int gdt, const void* G, long long ldg, int n, int K);
 constexpr const char* kFusedVariant = "bf16x3_fused_256x256";
 const char* gemm_variant_name(int prec, int M, int N, int K);
+// shape classes timed by the autotuner in this process
+int gemm_tunings();
 int gemm_row_pad();
 // A rows read by launch_gemm_fixed: M rounded up to its 128-row tiles
 int gemm_fixed_row_pad();

@@ -87,7 +87,7 @@ class Request(ctypes.Structure):
 EXPORTS = ["oagpu_create", "oagpu_destroy", "oagpu_last_error", "oagpu_set_subset", "oagpu_set_covariates",
            "oagpu_set_interactions", "oagpu_set_dosage_model", "oagpu_load_traits", "oagpu_load_genotypes",
            "oagpu_scan", "oagpu_scan_async", "oagpu_scan_wait", "oagpu_host_alloc", "oagpu_host_free",
-           "oagpu_last_timings", "oagpu_gemm_variant", "oagpu_tvalue",
+           "oagpu_last_timings", "oagpu_gemm_variant", "oagpu_gemm_tunings", "oagpu_tvalue",
            "oagpu_panel_create", "oagpu_panel_destroy", "oagpu_server_create", "oagpu_server_destroy",
            "oagpu_server_last_error", "oagpu_server_graph_chunks", "oagpu_server_submit", "oagpu_server_poll", "oagpu_server_wait",
            "oagpu_scan_noalloc"]
@@ -131,6 +131,8 @@ def load():
     L.oagpu_last_timings.argtypes = [vp, ctypes.POINTER(ctypes.c_double), i]
     L.oagpu_gemm_variant.argtypes = [vp]
     L.oagpu_gemm_variant.restype = ctypes.c_char_p
+    L.oagpu_gemm_tunings.argtypes = []
+    L.oagpu_gemm_tunings.restype = ctypes.c_int
     L.oagpu_tvalue.argtypes = [ctypes.c_float]
     L.oagpu_tvalue.restype = ctypes.c_double
     L.oagpu_panel_create.argtypes = [i, i, ll, vp, i, i, ll, ctypes.POINTER(vp)]

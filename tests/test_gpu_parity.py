@@ -130,7 +130,8 @@ def test_mixed_precision_bounds(precision, oracle_mod):
 
 
 GEMM_VARIANTS = {
-    "fp64": ["f64_direct_128x128"],
+    "fp64": ["f64_direct_128x128", "f64_direct_128x128_k32", "f64_direct_128x64", "f64_direct_64x128",
+             "f64_direct_64x64_k32"],
     "fp32": ["f32_direct_128x128", "f32_hipblaslt"],
     "bf16x3": ["bf16x3_hipblaslt", "bf16x3_lds_pf2_128x128", "bf16x3_lds_256x128", "bf16x3_lds_128x128",
                "bf16x3_direct_128x128", "bf16x3_fused_256x256"],
@@ -351,3 +352,37 @@ def test_fallback_arena_overflow(miss, oracle_mod, monkeypatch):
     want = hits_from_dense(run_oracle(oracle_mod, data, **thr), data.m)
     got = np.stack([gs.hits["trait"], gs.hits["snp"], gs.hits["h"]], axis=1)
     np.testing.assert_array_equal(got, want)
+
+
+def test_tuning_cache_persists(tmp_path):
+    """The contraction autotuner's choice for a shape class is written to the
+    cache file and read by the next process, which then times nothing."""
+    import subprocess
+    import sys
+    import textwrap
+    cache = tmp_path / "tune.txt"
+    prog = textwrap.dedent("""
+        import sys
+        sys.path.insert(0, %r)
+        sys.path.insert(0, %r)
+        from omicabelnomm_amd import synth, lib
+        from parity_util import run_gpu
+        data = synth.make(n=640, l=3, m=90, t=70, miss="mcar", seed=21)
+        g = run_gpu(data, "fp64", dense=False)
+        print("TUNINGS", lib.load().oagpu_gemm_tunings(), g.stats["n_hits"])
+    """ % (str(__import__("pathlib").Path(__file__).resolve().parents[1]),
+           str(__import__("pathlib").Path(__file__).resolve().parent)))
+    env = dict(__import__("os").environ, OAGPU_TUNE_CACHE=str(cache))
+    env.pop("OAGPU_GEMM_VARIANT", None)
+    outs = []
+    for run in range(2):
+        print("case tuning cache run", run, flush=True)
+        p = subprocess.run([sys.executable, "-c", prog], env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                           text=True, timeout=300)
+        assert p.returncode == 0, p.stdout
+        line = [ln for ln in p.stdout.splitlines() if ln.startswith("TUNINGS")][0].split()
+        outs.append((int(line[1]), int(line[2])))
+    lines = cache.read_text().split("\n")
+    assert any(ln.split()[-1].startswith("f64_direct") for ln in lines if ln.strip()), lines
+    assert outs[0][0] >= 1 and outs[1][0] == 0, outs     # the second process times nothing
+    assert outs[0][1] == outs[1][1]
