@@ -166,7 +166,7 @@ DevScan oa::make_dev(const oagpu_ctx* c) {
   d.Xu = c->Xu; d.xu_dt = xu_dtype(c); d.ldxu = c->ldxu; d.pst = c->pst;
   d.Gm = c->Gm; d.gdt = c->gdt; d.ldg = c->ldg; d.col_src = c->col_src; d.col_env = c->col_env;
   d.g_vec = c->Gm && ((uintptr_t)c->Gm % 16 == 0) && (c->ldg % 4 == 0);
-  d.col_mean = c->col_mean; d.Sbr = c->Sbr; d.Str = c->Str; d.varx = c->varx; d.Ccorr = c->Ccorr;
+  d.col_mean = c->col_mean; d.col_nan = c->col_nan; d.Sbr = c->Sbr; d.Str = c->Str; d.varx = c->varx; d.Ccorr = c->Ccorr;
   d.Cbuf = c->Cbuf;
   d.hits = c->hits; d.hit_cap = c->hit_cap; d.counters = c->counters;
   d.trait_offset = c->trait_offset; d.snp_offset = c->snp_offset; d.m_total = c->m;
@@ -272,7 +272,7 @@ extern "C" void oagpu_destroy(oagpu_ctx* c) {
                   c->Y_own, c->Bbuf, c->rows, c->ymiss, c->yzero, c->ypat, c->yy, c->sst,
                   c->rss0, c->rt, c->zt, c->ay, c->hash, c->scratch_i, c->pat_cnt, c->pat_rep, c->pat_pd, c->keep,
                   c->UA, c->UA_invd, c->wtrait, c->wpat, c->Abuf, c->Cbuf,
-                  c->col_mean, c->Sbr, c->Str, c->varx, c->Ccorr, c->hits, c->counters, c->fb,
+                  c->col_mean, c->col_nan, c->Sbr, c->Str, c->varx, c->Ccorr, c->hits, c->counters, c->fb,
                   c->uidx, c->pu_off, c->pu_list, c->XLu, c->Xu, c->pst};
   for (void* b : bufs) dfree(b);
   if (c->st) (void)hipStreamDestroy(c->st);
@@ -763,8 +763,10 @@ int oa::ensure_pair_buffers(oagpu_ctx* c, int mc) {
   const int r = c->r, l = c->l, p = c->p, G = c->G, Q = c->Q;
   const size_t qbytes = sizeof(double) * (size_t)mc * Q * std::max(G, 1);
   if (mc > c->mc_cap) {
-    dfree(c->col_mean); dfree(c->Sbr); dfree(c->Str); dfree(c->varx);
+    dfree(c->col_mean); dfree(c->col_nan); dfree(c->Sbr); dfree(c->Str); dfree(c->varx);
+    c->col_nan = nullptr;
     HIPCK(c, hipMalloc(&c->col_mean, sizeof(double) * (size_t)mc * r));
+    HIPCK(c, hipMalloc(&c->col_nan, sizeof(int) * (size_t)mc * r));
     HIPCK(c, hipMalloc(&c->Sbr, sizeof(double) * (size_t)mc * (r * (r + 1) / 2)));
     HIPCK(c, hipMalloc(&c->Str, sizeof(double) * (size_t)mc * l * r));
     HIPCK(c, hipMalloc(&c->varx, sizeof(double) * (size_t)mc * p));
@@ -827,6 +829,7 @@ static FusedGemmArgs fused_args(const oagpu_ctx* c, const DevScan& d, int M) {
   a.col_src = d.col_src; a.s0 = d.s0;
   a.E = c->E32 + (size_t)c->pair_env * c->n_pad;
   a.col_mean = d.col_mean;
+  a.col_nan = d.col_nan;
   a.B = static_cast<const uint16_t*>(c->Blo); a.ldb = c->ldop;
   a.C = static_cast<float*>(const_cast<void*>(d.Cbuf)); a.ldc = out_cols(c);
   a.M = M; a.N = out_cols(c);

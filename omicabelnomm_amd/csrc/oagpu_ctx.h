@@ -120,6 +120,7 @@ struct oagpu_ctx {
   void* Abuf = nullptr;
   void *Ahi = nullptr, *Alo = nullptr, *Cbuf = nullptr;   // Ahi, Alo: views into Abuf
   double *col_mean = nullptr, *Sbr = nullptr, *Str = nullptr, *varx = nullptr, *Ccorr = nullptr;
+  int* col_nan = nullptr;   // per chunk column: 1 if it holds a missing genotype
   void* Xu = nullptr;   // FP64, or FP32 below an FP64 contraction (xu_dtype)
   long long ldxu = 0;
   // few patterns per trait block: per-(SNP, pattern) solve pieces

@@ -92,6 +92,7 @@ struct DevScan {
   // of one size, the start written before each replay
   const int* s0_dev;
   double* col_mean;      // [mc*r]
+  int* col_nan;          // [mc*r] 1 if the column has a missing value (written when set)
   double* Sbr;           // [mc][r(r+1)/2] packed upper
   double* Str;           // [mc][l][r]
   double* varx;          // [mc][p]

@@ -21,9 +21,13 @@
 // k = 8g..8g+7 of row i for both operands.
 //
 // LDS: two stages, each operand's hi and lo halves in separate arrays of
-// 64-byte rows whose 16-byte chunks are XOR-swizzled by (row >> 1) & 3 (the
-// MFMA operand reads and the B fills are then free of bank conflicts; a
-// padded 80-byte row measured 1.33x slower on gfx950).  Per step s:
+// 64-byte rows.  B rows' 16-byte chunks are XOR-swizzled by (row >> 1) & 3;
+// the A stage keeps the tile's E*G rows in rows 0..127 and its G rows in
+// 128..255 with its own swizzle (swa), so the conversion's stores fill
+// consecutive rows: the operand reads, the B fills and the conversion's
+// stores are all free of bank conflicts (the interleaved A rows cost 1.0e9
+// conflict cycles per n = 10,000 launch; a padded 80-byte row measured
+// 1.33x slower).  Per step s:
 //   1. convert the genotypes of step s+1 (loaded two steps earlier) into the
 //      other stage;
 //   2. DMA B's k-slice of step s+1 into the other stage (global_load_lds,
@@ -32,9 +36,14 @@
 //   4. the MFMAs of step s;
 //   5. wait for this wave's DMA and LDS writes (the 3 newer loads may stay in
 //      flight), barrier.
-// Converting before the DMA issue matters: the compiler does not see the
-// DMA (issued from asm, so it adds no conservative wait before LDS reads),
-// and its wait for the genotype registers then covers only older DMAs.
+// Waves 4..7 run 4 before 1-3 (each SIMD holds one wave of each order, so
+// one converts while the other multiplies).  Converting before the DMA issue
+// matters for waves 0..3: the compiler does not see the DMA (issued from
+// asm, so it adds no conservative wait before LDS reads), and its wait for
+// the genotype registers then covers only older DMAs; in waves 4..7 that
+// wait also covers the step's own DMA, issued a whole MFMA phase earlier.
+// A tile whose columns hold no missing genotype runs a loop without the
+// mean substitution (4 of the ~11 VALU operations per genotype).
 // Genotype runs past n re-read the column's last run: B is zero there.
 //
 // Workgroups map to tiles XCD-major with the B tiles of one A row block
@@ -63,6 +72,16 @@ __device__ __forceinline__ int xcd_tile_f(int id, int total) {
 
 // bf16 offset of 16-byte chunk c of row r (64-byte rows, chunk xor (r >> 1))
 __device__ __forceinline__ int sw(int r, int c) { return r * 32 + (((c ^ (r >> 1)) & 3) << 3); }
+
+// The A stages hold the E*G rows of the tile's 128 SNPs in rows 0..127 and
+// the G rows in 128..255 (MFMA row 2s + t of the tile is stage row 128 t +
+// s), so the conversion's 8-byte stores of a wave fill consecutive rows.
+// Chunk c of stage row R sits at c xor (2 ((R >> 1) & 1) + (R >> 7)): the
+// 8 lanes an LDS cycle serves for an operand read (4 SNPs, both rows of
+// each) then hit 8 different 16-byte bank groups.
+__device__ __forceinline__ int swa(int R, int c) {
+  return R * 32 + ((c ^ ((((R >> 1) & 1) << 1) | ((R >> 7) & 1))) << 3);
+}
 
 // 16 bytes per lane from global memory into LDS at lds + 16 * lane.  From
 // asm: no output registers; the caller waits (vmcnt) before publishing.
@@ -113,13 +132,18 @@ __global__ __launch_bounds__(512, 1) void k_gemm_bf16x3_fused(FusedGemmArgs a) {
   const int ms = a.M >> 1;
   const GT* gcol[2];
   float mE[2], mG[2];
+  int tnan = 0;
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
     const int s = min(tm * 128 + col0 + 64 * j, ms - 1);
     gcol[j] = static_cast<const GT*>(a.G) + (size_t)a.col_src[(size_t)(a.s0 + s) * 2] * a.ldg;
     mE[j] = (float)a.col_mean[2 * s];
     mG[j] = (float)a.col_mean[2 * s + 1];
+    tnan |= a.col_nan[2 * s + 1];
   }
+  // missing genotypes in the tile's columns: the conversion substitutes
+  // means (workgroup-uniform choice of the loop below)
+  const bool anynan = __syncthreads_or(tnan) != 0;
   // B DMA: wave w fills rows 16 (4 (w & 3) + q) .. +15 of half w >> 2 (1: hi
   // at +K in memory, 0: lo); lane -> row + lane / 4, physical chunk lane & 3
   const int bpart = w >> 2;
@@ -145,7 +169,7 @@ __global__ __launch_bounds__(512, 1) void k_gemm_bf16x3_fused(FusedGemmArgs a) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) dma16(bbase + bsrc[q] + k0, &SB(ST)[bpart ? 0 : 1][(16 * (4 * (w & 3) + q)) * 32]);
   };
-  auto conv = [&](auto ST, const GE& P) {
+  auto conv = [&](auto NANP, auto ST, const GE& P) {
     const float ee[4] = {P.e.x, P.e.y, P.e.z, P.e.w};
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
@@ -155,8 +179,13 @@ __global__ __launch_bounds__(512, 1) void k_gemm_bf16x3_fused(FusedGemmArgs a) {
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const float ve = gg[e] * ee[e];
-        xe[e] = (ve != ve) ? mE[j] : ve;
-        xg[e] = (gg[e] != gg[e]) ? mG[j] : gg[e];
+        if constexpr (decltype(NANP)::value) {
+          xe[e] = (ve != ve) ? mE[j] : ve;
+          xg[e] = (gg[e] != gg[e]) ? mG[j] : gg[e];
+        } else {
+          xe[e] = ve;
+          xg[e] = gg[e];
+        }
       }
       unsigned h[4], l[4];
 #pragma unroll
@@ -172,8 +201,8 @@ __global__ __launch_bounds__(512, 1) void k_gemm_bf16x3_fused(FusedGemmArgs a) {
         h[2 + p] = __builtin_bit_cast(unsigned, hu);
         l[2 + p] = __builtin_bit_cast(unsigned, lu);
       }
-      const int row = 2 * (col0 + 64 * j);
-      const int o0 = sw(row, gc >> 1) + 4 * (gc & 1), o1 = sw(row + 1, gc >> 1) + 4 * (gc & 1);
+      const int snp = col0 + 64 * j;
+      const int o0 = swa(snp, gc >> 1) + 4 * (gc & 1), o1 = swa(128 + snp, gc >> 1) + 4 * (gc & 1);
       *reinterpret_cast<uint2*>(&SA(ST)[0][o0]) = make_uint2(h[0], h[1]);
       *reinterpret_cast<uint2*>(&SA(ST)[1][o0]) = make_uint2(l[0], l[1]);
       *reinterpret_cast<uint2*>(&SA(ST)[0][o1]) = make_uint2(h[2], h[3]);
@@ -195,7 +224,8 @@ __global__ __launch_bounds__(512, 1) void k_gemm_bf16x3_fused(FusedGemmArgs a) {
     }
 #pragma unroll
     for (int x = 0; x < 8; ++x) {
-      const int o = sw(wr + 16 * x + i, g);
+      const int L = wr + 16 * x + i;   // MFMA row of the tile: SNP L / 2, row type L & 1
+      const int o = swa(((L & 1) << 7) + (L >> 1), g);
       const u4 ah = *reinterpret_cast<const u4*>(&SA(CUR)[0][o]);
       const u4 al = *reinterpret_cast<const u4*>(&SA(CUR)[1][o]);
 #pragma unroll
@@ -209,33 +239,56 @@ __global__ __launch_bounds__(512, 1) void k_gemm_bf16x3_fused(FusedGemmArgs a) {
         acc[x][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_b8(ah), as_b8(bh[b]), acc[x][b], 0, 0, 0);
     }
   };
-  GE P, Q;
-  {
-    GE p0;
-    geload(0, p0);
-    conv(std::integral_constant<int, 0>{}, p0);
-    bdma(std::integral_constant<int, 0>{}, 0);
-  }
-  geload(1, P);
-  geload(2, Q);
-  // the step-0 DMA and LDS writes done; the 6 loads of P and Q may fly
-  __asm__ volatile("s_waitcnt vmcnt(6) lgkmcnt(0)\n\ts_barrier" ::: "memory");
-  auto step = [&](auto CUR, int s, GE& Pc) {
-    constexpr int cur = decltype(CUR)::value;
-    std::integral_constant<int, cur ^ 1> NXT;
-    conv(NXT, Pc);          // genotypes of step s+1
-    bdma(NXT, s + 1);       // B of step s+1 (the other stage is free since the last barrier)
-    geload(s + 3, Pc);
-    consume(CUR);
-    // this wave's DMA and LDS writes done; its 3 newer loads may fly
-    __asm__ volatile("s_waitcnt vmcnt(3) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+  // Waves 4..7 share their SIMDs with waves 0..3 and run each step's MFMAs
+  // before its conversion, the others after it, so on every SIMD one wave
+  // converts (VALU) while the other multiplies (MFMA).
+  const bool late = w >= 4;
+  auto run = [&](auto NANP, auto LATE) {
+    std::integral_constant<int, 0> S0;
+    std::integral_constant<int, 1> S1;
+    GE P, Q;
+    {
+      GE p0;
+      geload(0, p0);
+      conv(NANP, S0, p0);
+      bdma(S0, 0);
+    }
+    geload(1, P);
+    geload(2, Q);
+    // the step-0 DMA and LDS writes done; the 6 loads of P and Q may fly
+    __asm__ volatile("s_waitcnt vmcnt(6) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    auto step = [&](auto CUR, int s, GE& Pc) {
+      constexpr int cur = decltype(CUR)::value;
+      std::integral_constant<int, cur ^ 1> NXT;
+      if constexpr (decltype(LATE)::value) {
+        bdma(NXT, s + 1);       // B of step s+1 (the other stage is free since the last barrier)
+        consume(CUR);
+        conv(NANP, NXT, Pc);    // genotypes of step s+1
+        geload(s + 3, Pc);
+      } else {
+        conv(NANP, NXT, Pc);
+        bdma(NXT, s + 1);
+        geload(s + 3, Pc);
+        consume(CUR);
+      }
+      // this wave's DMA and LDS writes done; its 3 newer loads may fly
+      __asm__ volatile("s_waitcnt vmcnt(3) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    };
+    int s = 0;
+    for (; s + 2 <= nsteps; s += 2) {
+      step(S0, s, P);
+      step(S1, s + 1, Q);
+    }
+    if (s < nsteps) step(S0, s, P);
   };
-  int s = 0;
-  for (; s + 2 <= nsteps; s += 2) {
-    step(std::integral_constant<int, 0>{}, s, P);
-    step(std::integral_constant<int, 1>{}, s + 1, Q);
+  // four copies of the loop, each branch-free inside
+  if (anynan) {
+    if (late) run(std::true_type{}, std::true_type{});
+    else run(std::true_type{}, std::false_type{});
+  } else {
+    if (late) run(std::false_type{}, std::true_type{});
+    else run(std::false_type{}, std::false_type{});
   }
-  if (s < nsteps) step(std::integral_constant<int, 0>{}, s, P);
   // lane (i, g), value v of block (x, b): A row 16x + 4g + v, B row 16b + i
 #pragma unroll
   for (int x = 0; x < 8; ++x)

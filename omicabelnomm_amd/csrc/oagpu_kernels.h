@@ -87,6 +87,7 @@ struct FusedGemmArgs {
   const int* col_src; int s0;        // block descriptors; SNP s reads column col_src[2 (s0 + s)]
   const float* E;                    // the interaction column, K floats, zero past n
   const double* col_mean;            // [M] chunk-local
+  const int* col_nan;                // [M] chunk-local: 1 if the column has a missing value
   const uint16_t* B; long long ldb;
   float* C; long long ldc;
   int M, N, tiles_n, total;

@@ -243,7 +243,10 @@ __global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo) {
     for (int q = 0; q < PREP_SPW; ++q)
       if (live[q])
 #pragma unroll
-        for (int c = 0; c < R; ++c) d.col_mean[(size_t)(sb + q) * R + c] = mean[q * R + c];
+        for (int c = 0; c < R; ++c) {
+          d.col_mean[(size_t)(sb + q) * R + c] = mean[q * R + c];
+          if (d.col_nan) d.col_nan[(size_t)(sb + q) * R + c] = red[NV + q * R + c] > 0.0 ? 1 : 0;
+        }
   }
 }
 

@@ -161,6 +161,7 @@ extern "C" int oagpu_scan_noalloc(oagpu_ctx* c, oagpu_hit* hits, long long hit_c
   d.fb_cap = kFallbackCap;
   d.Cbuf = L.C;
   d.col_mean = L.col_mean;
+  d.col_nan = nullptr;   // only the fused contraction reads the flags
   d.Sbr = L.Sbr;
   d.Str = L.Str;
   d.varx = L.varx;
