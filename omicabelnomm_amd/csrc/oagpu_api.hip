@@ -273,7 +273,7 @@ extern "C" void oagpu_destroy(oagpu_ctx* c) {
                   c->rss0, c->rt, c->zt, c->ay, c->hash, c->scratch_i, c->pat_cnt, c->pat_rep, c->pat_pd, c->keep,
                   c->UA, c->UA_invd, c->wtrait, c->wpat, c->Abuf, c->Cbuf,
                   c->col_mean, c->col_nan, c->Sbr, c->Str, c->varx, c->Ccorr, c->hits, c->counters, c->fb,
-                  c->uidx, c->pu_off, c->pu_list, c->XLu, c->Xu, c->pst};
+                  c->uidx, c->pu_off, c->pu_list, c->XLu, c->Xu, c->pst, c->Bf};
   for (void* b : bufs) dfree(b);
   if (c->st) (void)hipStreamDestroy(c->st);
   if (c->cst) (void)hipStreamDestroy(c->cst);
@@ -584,6 +584,7 @@ extern "C" int oagpu_load_traits(oagpu_ctx* c, const void* Y, int dtype, int whe
   HIPCK(c, hipGetLastError());
   HIPCK(c, hipStreamSynchronize(c->st));
   c->have_traits = true;
+  c->bf_ok = false;
   return OAGPU_OK;
 }
 
@@ -830,7 +831,7 @@ static FusedGemmArgs fused_args(const oagpu_ctx* c, const DevScan& d, int M) {
   a.E = c->E32 + (size_t)c->pair_env * c->n_pad;
   a.col_mean = d.col_mean;
   a.col_nan = d.col_nan;
-  a.B = static_cast<const uint16_t*>(c->Blo); a.ldb = c->ldop;
+  a.B = c->Bf; a.ldb = c->ldop;
   a.C = static_cast<float*>(const_cast<void*>(d.Cbuf)); a.ldc = out_cols(c);
   a.M = M; a.N = out_cols(c);
   return a;
@@ -860,6 +861,13 @@ int oa::enqueue_chunks(oagpu_ctx* c, DevScan& d, long long mc, void* Ahi, void* 
     };
     bool fused = false;
     if (!fixed_gemm && fused_capable(c)) {
+      if (!c->bf_ok) {
+        const int rows = out_cols(c);
+        HIPCK(c, grow((void**)&c->Bf, &c->Bf_cap, (size_t)rows * c->ldop * sizeof(uint16_t)));
+        launch_fused_b_layout(static_cast<const uint16_t*>(c->Blo), c->ldop, c->n_pad, rows, c->Bf, c->st);
+        HIPCK(c, hipGetLastError());
+        c->bf_ok = true;
+      }
       const char* pin = getenv("OAGPU_GEMM_VARIANT");
       if (pin) {
         fused = !strcmp(pin, kFusedVariant);

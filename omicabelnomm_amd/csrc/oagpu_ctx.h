@@ -52,6 +52,11 @@ struct oagpu_ctx {
   size_t Y_own_bytes = 0;
   void* Bbuf = nullptr;
   void *Bhi = nullptr, *Blo = nullptr;   // views into Bbuf
+  // the fused contraction's copy of B (launch_fused_b_layout), built on
+  // first use after the traits were loaded
+  uint16_t* Bf = nullptr;
+  size_t Bf_cap = 0;
+  bool bf_ok = false;
   int* rows = nullptr;
   int *ymiss = nullptr, *yzero = nullptr, *ypat = nullptr;
   double *yy = nullptr, *sst = nullptr, *rss0 = nullptr, *rt = nullptr, *zt = nullptr, *ay = nullptr;

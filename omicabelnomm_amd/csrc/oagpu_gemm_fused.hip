@@ -20,9 +20,12 @@
 // hi*lo, hi*hi) in FP32.  Lane (i, g) = (lane & 15, lane >> 4) supplies
 // k = 8g..8g+7 of row i for both operands.
 //
-// LDS: two stages, each operand's hi and lo halves in separate arrays of
-// 64-byte rows.  B rows' 16-byte chunks are XOR-swizzled by (row >> 1) & 3;
-// the A stage keeps the tile's E*G rows in rows 0..127 and its G rows in
+// LDS: two stages.  B arrives in its own copy (launch_fused_b_layout) where
+// a row's hi and lo values of one k-step are one 128-byte run, so each
+// k-step fetches whole cache lines (the interleaved [lo | hi] rows cost a
+// half-used line per half: 1.8 % slower); its stage rows are those 128
+// bytes with the 16-byte chunks XOR-swizzled by row & 7.  A's hi and lo
+// halves are separate arrays of 64-byte rows; the A stage keeps the tile's E*G rows in rows 0..127 and its G rows in
 // 128..255 with its own swizzle (swa), so the conversion's stores fill
 // consecutive rows: the operand reads, the B fills and the conversion's
 // stores are all free of bank conflicts (the interleaved A rows cost 1.0e9
@@ -49,6 +52,7 @@
 // Workgroups map to tiles XCD-major with the B tiles of one A row block
 // consecutive, so an XCD works on a few row blocks x all column blocks at a
 // time and each genotype column and B k-slice is fetched into its L2 once.
+#include <algorithm>
 #include <type_traits>
 
 #include "oagpu_kernels.h"
@@ -115,11 +119,12 @@ template <typename GT>
 __global__ __launch_bounds__(512, 1) void k_gemm_bf16x3_fused(FusedGemmArgs a) {
   typedef typename Run4<GT>::T RV;
   __shared__ __attribute__((aligned(16))) uint16_t sA0[2][FT * 32], sA1[2][FT * 32];   // [hi, lo]
-  __shared__ __attribute__((aligned(16))) uint16_t sB0[2][FT * 32], sB1[2][FT * 32];
+  // B stage: 128-byte rows [hi | lo] of the k-step (fused_b_layout)
+  __shared__ __attribute__((aligned(16))) uint16_t sB0[FT * 64], sB1[FT * 64];
   auto SA = [&](auto ST) -> uint16_t(*)[FT * 32] {
     if constexpr (decltype(ST)::value == 0) return sA0; else return sA1;
   };
-  auto SB = [&](auto ST) -> uint16_t(*)[FT * 32] {
+  auto SB = [&](auto ST) -> uint16_t* {
     if constexpr (decltype(ST)::value == 0) return sB0; else return sB1;
   };
   const int tile = xcd_tile_f(blockIdx.x, a.total);
@@ -144,15 +149,15 @@ __global__ __launch_bounds__(512, 1) void k_gemm_bf16x3_fused(FusedGemmArgs a) {
   // missing genotypes in the tile's columns: the conversion substitutes
   // means (workgroup-uniform choice of the loop below)
   const bool anynan = __syncthreads_or(tnan) != 0;
-  // B DMA: wave w fills rows 16 (4 (w & 3) + q) .. +15 of half w >> 2 (1: hi
-  // at +K in memory, 0: lo); lane -> row + lane / 4, physical chunk lane & 3
-  const int bpart = w >> 2;
+  // B DMA: wave w fills rows 32w + 8q .. +7 of the stage; lane -> row
+  // + lane / 8, physical 16-byte chunk lane & 7 (chunks 0..3 hi, 4..7 lo,
+  // chunk c of row r stored at c xor (r & 7))
   int bsrc[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const int r = 16 * (4 * (w & 3) + q) + (lane >> 2);
-    const int logical = ((lane & 3) ^ (r >> 1)) & 3;
-    bsrc[q] = (min(tn * FT + r, a.N - 1) - tn * FT) * (int)a.ldb + bpart * a.K + 8 * logical;
+    const int r = 32 * w + 8 * q + (lane >> 3);
+    const int logical = (lane & 7) ^ (r & 7);
+    bsrc[q] = (min(tn * FT + r, a.N - 1) - tn * FT) * (int)a.ldb + 8 * logical;
   }
   const uint16_t* bbase = a.B + (size_t)(tn * FT) * a.ldb;
   const int nsteps = a.K / FK;
@@ -165,9 +170,9 @@ __global__ __launch_bounds__(512, 1) void k_gemm_bf16x3_fused(FusedGemmArgs a) {
     r.e = *reinterpret_cast<const float4*>(a.E + kk);
   };
   auto bdma = [&](auto ST, int step) {
-    const int k0 = min(step, nsteps - 1) * FK;
+    const int k0 = min(step, nsteps - 1) * 2 * FK;   // [hi | lo] of each k-step
 #pragma unroll
-    for (int q = 0; q < 4; ++q) dma16(bbase + bsrc[q] + k0, &SB(ST)[bpart ? 0 : 1][(16 * (4 * (w & 3) + q)) * 32]);
+    for (int q = 0; q < 4; ++q) dma16(bbase + bsrc[q] + k0, SB(ST) + (32 * w + 8 * q) * 64);
   };
   auto conv = [&](auto NANP, auto ST, const GE& P) {
     const float ee[4] = {P.e.x, P.e.y, P.e.z, P.e.w};
@@ -218,9 +223,9 @@ __global__ __launch_bounds__(512, 1) void k_gemm_bf16x3_fused(FusedGemmArgs a) {
     u4 bh[4], bl[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-      const int o = sw(wc + 16 * b + i, g);
-      bh[b] = *reinterpret_cast<const u4*>(&SB(CUR)[0][o]);
-      bl[b] = *reinterpret_cast<const u4*>(&SB(CUR)[1][o]);
+      const int r = wc + 16 * b + i;
+      bh[b] = *reinterpret_cast<const u4*>(SB(CUR) + r * 64 + ((g ^ (r & 7)) << 3));
+      bl[b] = *reinterpret_cast<const u4*>(SB(CUR) + r * 64 + (((4 + g) ^ (r & 7)) << 3));
     }
 #pragma unroll
     for (int x = 0; x < 8; ++x) {
@@ -301,6 +306,25 @@ __global__ __launch_bounds__(512, 1) void k_gemm_bf16x3_fused(FusedGemmArgs a) {
         if (row < a.M && col < a.N) a.C[(size_t)row * a.ldc + col] = acc[x][b][v];
       }
     }
+}
+
+// The fused contraction's copy of B: per row, each k-step's 32 hi values then
+// its 32 lo values (one 128-byte run), from rows interleaved [lo | hi].
+__global__ __launch_bounds__(256) void k_fused_b_layout(const uint16_t* B, long long ldb, int K, int rows,
+                                                        uint16_t* Bf) {
+  const long long chunks = (long long)rows * (K / 4);   // 16-byte output chunks (2K values per row)
+  for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < chunks; q += (long long)gridDim.x * 256) {
+    const int j = (int)(q / (K / 4)), o = (int)(q - (long long)j * (K / 4));   // output chunk o of row j
+    const int st = o >> 3, c = o & 7;                                          // k-step, chunk within it
+    const uint16_t* src = B + (size_t)j * ldb + (c < 4 ? K : 0) + st * FK + (c & 3) * 8;
+    *reinterpret_cast<u4*>(Bf + (size_t)j * ldb + (size_t)o * 8) = *reinterpret_cast<const u4*>(src);
+  }
+}
+
+void launch_fused_b_layout(const uint16_t* B, long long ldb, int K, int rows, uint16_t* Bf, hipStream_t st) {
+  const long long chunks = (long long)rows * (K / 4);
+  const int blocks = (int)std::min<long long>((chunks + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fused_b_layout, dim3(blocks), dim3(256), 0, st, B, ldb, K, rows, Bf);
 }
 
 bool gemm_fused_supported(int gdt, const void* G, long long ldg, int n, int K) {

@@ -88,11 +88,16 @@ struct FusedGemmArgs {
   const float* E;                    // the interaction column, K floats, zero past n
   const double* col_mean;            // [M] chunk-local
   const int* col_nan;                // [M] chunk-local: 1 if the column has a missing value
-  const uint16_t* B; long long ldb;
+  const uint16_t* B; long long ldb;  // launch_fused_b_layout rows
   float* C; long long ldc;
   int M, N, tiles_n, total;
 };
 void launch_gemm_fused(const FusedGemmArgs& a, int gdt, hipStream_t st);
+// The B operand the fused contraction reads (FusedGemmArgs::B): rows of B
+// (interleaved [lo | hi], stride ldb = 2K) rewritten with each 32-value
+// k-step's hi then lo values adjacent, so a k-step of a row is one 128-byte
+// run.  Same size and stride.
+void launch_fused_b_layout(const uint16_t* B, long long ldb, int K, int rows, uint16_t* Bf, hipStream_t st);
 // genotypes of element type gdt at G (column stride ldg, n rows, operand
 // depth K) the fused contraction takes: float32 or float16, 4-element runs
 // aligned, n % 4 == 0
