@@ -106,16 +106,18 @@ __global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo) {
     const int gs = d.s0 + (live[q] ? sb + q : sb);
     col_desc<R>(d, gs, src + q * R, env + q * R);
   }
-  // One pass that assumes no missing genotype (the common case): missing
-  // values enter as 0 while the columns' sums and NaN counts accumulate.  A
-  // block whose columns do hold NaN repeats the pass with the means imputed,
-  // overwriting everything the first pass wrote.
+  // One pass that assumes no missing genotype (the common case): values go
+  // into the sums unchecked, so a column with a missing value ends with a
+  // NaN sum.  Only then does the block count its missing values (sums over
+  // the others) and repeat the pass with the means imputed, overwriting
+  // everything the first pass wrote.
   double sum[NV], cnt[NV], mean[NV], acc[PREP_SPW * RR];
 #pragma unroll
   for (int v = 0; v < NV; ++v) { sum[v] = 0.0; cnt[v] = 0.0; mean[v] = 0.0; }
   // KEEP: a subset is set (a separate copy of the loop, so that without a
   // subset no flag load sits in front of the genotype loads)
-  auto run = [&](bool impute, auto KEEP) {
+  // CHECK: test every value for NaN (counting pass, imputing pass)
+  auto run = [&](bool impute, auto KEEP, auto CHECK) {
 #pragma unroll
     for (int e = 0; e < PREP_SPW * RR; ++e) acc[e] = 0.0;
     // A run's loads (subset flags, row-union positions, environment rows by
@@ -165,11 +167,15 @@ __global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo) {
           xv *= ev[e];   // 1 without an environment row
           if (!in[e]) {
             xv = 0.0;
-          } else if (xv == xv) {
-            if (!impute) sum[v] += xv;
+          } else if constexpr (decltype(CHECK)::value) {
+            if (xv == xv) {
+              if (!impute) sum[v] += xv;
+            } else {
+              if (!impute) cnt[v] += 1.0;
+              xv = mean[v];
+            }
           } else {
-            if (!impute) cnt[v] += 1.0;
-            xv = mean[v];
+            sum[v] += xv;
           }
           x[v][e] = xv;
         }
@@ -207,20 +213,35 @@ __global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo) {
       }
     }
   };
-  if (d.keep) run(false, std::true_type{});
-  else run(false, std::false_type{});
+  std::false_type fast;
+  std::true_type checked;
+  if (d.keep) run(false, std::true_type{}, fast);
+  else run(false, std::false_type{}, fast);
   // one block reduction for the column sums, NaN counts and Gram pieces; the
   // Gram pieces are final unless some column holds NaN
   constexpr int NA = PREP_SPW * RR;
   double red[2 * NV + NA];
+  auto reduce = [&]() {
 #pragma unroll
-  for (int v = 0; v < NV; ++v) {
-    red[v] = sum[v];
-    red[NV + v] = cnt[v];
+    for (int v = 0; v < NV; ++v) {
+      red[v] = sum[v];
+      red[NV + v] = cnt[v];
+    }
+#pragma unroll
+    for (int e = 0; e < NA; ++e) red[2 * NV + e] = acc[e];
+    block_sum256_n<2 * NV + NA>(red, shd);
+  };
+  reduce();
+  bool nan_sum = false;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) nan_sum = nan_sum || (live[v / R] && red[v] != red[v]);
+  if (nan_sum) {   // block-uniform: count the missing values, sum the others
+#pragma unroll
+    for (int v = 0; v < NV; ++v) sum[v] = 0.0;
+    if (d.keep) run(false, std::true_type{}, checked);
+    else run(false, std::false_type{}, checked);
+    reduce();
   }
-#pragma unroll
-  for (int e = 0; e < NA; ++e) red[2 * NV + e] = acc[e];
-  block_sum256_n<2 * NV + NA>(red, shd);
   double missing = 0.0;
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
@@ -228,8 +249,8 @@ __global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo) {
     if (live[v / R]) missing += red[NV + v];
   }
   if (missing > 0.0) {   // block-uniform; each thread rewrites its own rows
-    if (d.keep) run(true, std::true_type{});
-    else run(true, std::false_type{});
+    if (d.keep) run(true, std::true_type{}, checked);
+    else run(true, std::false_type{}, checked);
     block_sum256_n<NA>(acc, shd);
   } else {
 #pragma unroll
