@@ -22,15 +22,17 @@
 //
 // LDS: two stages.  B arrives in its own copy (launch_fused_b_layout) where
 // a row's hi and lo values of one k-step are one 128-byte run, so each
-// k-step fetches whole cache lines (the interleaved [lo | hi] rows cost a
-// half-used line per half: 1.8 % slower); its stage rows are those 128
-// bytes with the 16-byte chunks XOR-swizzled by row & 7.  A's hi and lo
-// halves are separate arrays of 64-byte rows; the A stage keeps the tile's E*G rows in rows 0..127 and its G rows in
-// 128..255 with its own swizzle (swa), so the conversion's stores fill
-// consecutive rows: the operand reads, the B fills and the conversion's
-// stores are all free of bank conflicts (the interleaved A rows cost 1.0e9
-// conflict cycles per n = 10,000 launch; a padded 80-byte row measured
-// 1.33x slower).  Per step s:
+// k-step fetches whole cache lines (the interleaved [lo | hi] rows fetched a
+// half-used line per half: 1.8 % slower, and 13 % more L2 misses); its
+// stage rows are those 128 bytes with the 16-byte chunks XOR-swizzled by
+// row & 7.  A's hi and lo halves are separate arrays of 64-byte rows, the
+// tile's E*G rows in rows 0..127 and its G rows in 128..255 with their own
+// swizzle (swa), so a wave's conversion stores fill consecutive rows.
+// SQ_LDS_BANK_CONFLICT reads 2.0e9 cycles per n = 10,000 launch with this
+// kernel whatever the A layout (interleaved rows with the original wave
+// order: 1.0e9; a layout that puts a SNP's two rows in different bank
+// halves: again 2.0e9, and slower); a padded 80-byte row measured 1.33x
+// slower.  Per step s:
 //   1. convert the genotypes of step s+1 (loaded two steps earlier) into the
 //      other stage;
 //   2. DMA B's k-slice of step s+1 into the other stage (global_load_lds,
@@ -39,12 +41,11 @@
 //   4. the MFMAs of step s;
 //   5. wait for this wave's DMA and LDS writes (the 3 newer loads may stay in
 //      flight), barrier.
-// Waves 4..7 run 4 before 1-3 (each SIMD holds one wave of each order, so
-// one converts while the other multiplies).  Converting before the DMA issue
-// matters for waves 0..3: the compiler does not see the DMA (issued from
-// asm, so it adds no conservative wait before LDS reads), and its wait for
-// the genotype registers then covers only older DMAs; in waves 4..7 that
-// wait also covers the step's own DMA, issued a whole MFMA phase earlier.
+// Converting before the DMA issue matters: the compiler does not see the
+// DMA (issued from asm, so it adds no conservative wait before LDS reads),
+// and its wait for the genotype registers then covers only older DMAs.
+// (Running 4 before 1-3 in half of the waves, so that the two waves of a
+// SIMD convert and multiply at different times, measured 3.5 % slower.)
 // A tile whose columns hold no missing genotype runs a loop without the
 // mean substitution (4 of the ~11 VALU operations per genotype).
 // Genotype runs past n re-read the column's last run: B is zero there.
@@ -244,11 +245,7 @@ __global__ __launch_bounds__(512, 1) void k_gemm_bf16x3_fused(FusedGemmArgs a) {
         acc[x][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_b8(ah), as_b8(bh[b]), acc[x][b], 0, 0, 0);
     }
   };
-  // Waves 4..7 share their SIMDs with waves 0..3 and run each step's MFMAs
-  // before its conversion, the others after it, so on every SIMD one wave
-  // converts (VALU) while the other multiplies (MFMA).
-  const bool late = w >= 4;
-  auto run = [&](auto NANP, auto LATE) {
+  auto run = [&](auto NANP) {
     std::integral_constant<int, 0> S0;
     std::integral_constant<int, 1> S1;
     GE P, Q;
@@ -265,17 +262,10 @@ __global__ __launch_bounds__(512, 1) void k_gemm_bf16x3_fused(FusedGemmArgs a) {
     auto step = [&](auto CUR, int s, GE& Pc) {
       constexpr int cur = decltype(CUR)::value;
       std::integral_constant<int, cur ^ 1> NXT;
-      if constexpr (decltype(LATE)::value) {
-        bdma(NXT, s + 1);       // B of step s+1 (the other stage is free since the last barrier)
-        consume(CUR);
-        conv(NANP, NXT, Pc);    // genotypes of step s+1
-        geload(s + 3, Pc);
-      } else {
-        conv(NANP, NXT, Pc);
-        bdma(NXT, s + 1);
-        geload(s + 3, Pc);
-        consume(CUR);
-      }
+      conv(NANP, NXT, Pc);    // genotypes of step s+1
+      bdma(NXT, s + 1);       // B of step s+1 (the other stage is free since the last barrier)
+      geload(s + 3, Pc);
+      consume(CUR);
       // this wave's DMA and LDS writes done; its 3 newer loads may fly
       __asm__ volatile("s_waitcnt vmcnt(3) lgkmcnt(0)\n\ts_barrier" ::: "memory");
     };
@@ -286,14 +276,9 @@ __global__ __launch_bounds__(512, 1) void k_gemm_bf16x3_fused(FusedGemmArgs a) {
     }
     if (s < nsteps) step(S0, s, P);
   };
-  // four copies of the loop, each branch-free inside
-  if (anynan) {
-    if (late) run(std::true_type{}, std::true_type{});
-    else run(std::true_type{}, std::false_type{});
-  } else {
-    if (late) run(std::false_type{}, std::true_type{});
-    else run(std::false_type{}, std::false_type{});
-  }
+  // two copies of the loop, each branch-free inside
+  if (anynan) run(std::true_type{});
+  else run(std::false_type{});
   // lane (i, g), value v of block (x, b): A row 16x + 4g + v, B row 16b + i
 #pragma unroll
   for (int x = 0; x < 8; ++x)
