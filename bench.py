@@ -47,7 +47,7 @@ def parse():
     ap.add_argument("--config", type=int, default=2,
                     help="0-2: one scan per step; 3: 64 batched uneven jobs; 4: Poisson request stream")
     ap.add_argument("--jobs", type=int, default=0, help="requests of configs 3/4 (0 = the config's count)")
-    ap.add_argument("--chunk", type=int, default=8192, help="server chunk (SNPs), configs 3/4")
+    ap.add_argument("--chunk", type=int, default=-1, help="server chunk (SNPs), configs 3/4 (-1: chosen by timing)")
     ap.add_argument("--budget-gib", type=float, default=160.0, help="server HBM budget, configs 3/4")
     ap.add_argument("--graphs", action="store_true",
                     help="configs 3/4: replay each chunk's launches from a captured HIP graph")
@@ -63,6 +63,7 @@ def parse():
                     help="configs 0-2: panel SNPs of the bounded request-stream leg (0 = skip it)")
     ap.add_argument("--serve-leg-jobs", type=int, default=32)
     ap.add_argument("--serve-leg-rate", type=float, default=20.0, help="offered requests/s of that leg")
+    ap.add_argument("--serve-leg-chunk", type=int, default=-1, help="server chunk of that leg (-1: chosen by timing)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--oracle-seconds", type=float, default=5.0,
                     help="FP64 oracle work behind the hit check (planted SNPs, then the first ones)")
@@ -327,7 +328,7 @@ def percentile(xs, q):
     return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if xs else None
 
 
-def serve_leg(m, jobs, rate, p_store, precision):
+def serve_leg(m, jobs, rate, p_store, precision, chunk=-1):
     """The latency half of the metric beside the throughput headline: the
     request workload of config 4 (float16 panel of 20,000 individuals,
     requests with 1-8,192 traits, 2-16 covariates, MCAR 0-5 %) on a panel of
@@ -347,7 +348,7 @@ def serve_leg(m, jobs, rate, p_store, precision):
     # the specs' arrivals are a unit-rate Poisson process: scale to `rate`
     base = synth.SERVE_CONFIGS[4]["rate"]
     arrivals = [sp.arrival * base / rate for sp in specs]
-    with Server(panel, precision, chunk_snps=8192) as srv:
+    with Server(panel, precision, chunk_snps=chunk) as srv:
         warm = synth.JobSpec(t=64, l=3, keep=None, miss_rate=0.02, arrival=0.0, seed=1)
         srv.wait(srv.submit(*warm.arrays(n), **thr))   # autotuner, code objects
         ids = [None] * len(specs)
@@ -365,6 +366,7 @@ def serve_leg(m, jobs, rate, p_store, precision):
         th.join()
         answers = [srv.wait(i) for i in ids]
         wall = time.perf_counter() - start
+        chunk_used = srv.chunk_snps()
     panel.close()
     del G
     lat = [a.latency_ms for a in answers]
@@ -374,7 +376,7 @@ def serve_leg(m, jobs, rate, p_store, precision):
             "requests": len(specs), "traits": sum(sp.t for sp in specs), "offered_rate_per_s": rate,
             "seconds": round(wall, 3), "tests_per_s": tests / wall,
             "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99), "max_ms": max(lat),
-            "precision": precision}
+            "precision": precision, "chunk_snps": chunk_used, "chunk_timed": chunk < 0}
 
 
 def bench_serve(args):
@@ -404,6 +406,8 @@ def bench_serve(args):
     tests = sum(sp.t for sp in specs) * m
     panel = Panel(G)
 
+    chunk_used = [args.chunk]
+
     def serve(arrivals):
         srv = Server(panel, args.precision, hbm_budget=int(args.budget_gib * 2**30), max_active=64,
                      chunk_snps=args.chunk, graphs=args.graphs)
@@ -425,6 +429,7 @@ def bench_serve(args):
         answers = [srv.wait(i) for i in ids]
         wall = time.perf_counter() - start
         replayed = srv.graph_chunks()
+        chunk_used[0] = srv.chunk_snps()
         srv.close()
         print("chunks replayed from graphs: %d" % replayed, file=sys.stderr, flush=True)
         return answers, wall
@@ -492,7 +497,7 @@ def bench_serve(args):
                                   "individual subsets %d-%d" % cfg["subset"] if cfg["subset"] else
                                   "all individuals"),
                    "thresholds": {"minPstore": args.p_store, "minPdisp": args.p_store, "minR2store": -10.0},
-                   "precision": args.precision, "chunk_snps": args.chunk, "gen_seconds": round(gen_s, 2),
+                   "precision": args.precision, "chunk_snps": chunk_used[0], "chunk_timed": args.chunk < 0, "gen_seconds": round(gen_s, 2),
                    "graphs": args.graphs,
                    "value_is": "all requests submitted at once: total tests / time to the last answer"},
         "latency_ms": {"p50": percentile(lat, 50), "p99": percentile(lat, 99), "max": max(lat),
@@ -580,7 +585,7 @@ def main():
     serving = None
     if args.serve_leg_m > 0:
         serving = serve_leg(args.serve_leg_m, args.serve_leg_jobs, args.serve_leg_rate, args.p_store,
-                            args.precision)
+                            args.precision, args.serve_leg_chunk)
         print("request stream: p50 %.0f ms p99 %.0f ms" % (serving["p50_ms"], serving["p99_ms"]),
               file=sys.stderr, flush=True)
     dosage = dosage_leg(data.n, args.dosage) if args.dosage else None

@@ -46,7 +46,9 @@ typedef struct oagpu_server_params {
     int precision;          /* OAGPU_FP64 / OAGPU_FP32 / OAGPU_BF16X3              */
     long long hbm_budget;   /* bytes for active requests (0 = 64 GiB)               */
     int max_active;         /* requests in one batch (0 = 64)                       */
-    int chunk_snps;         /* SNPs per chunk (0 = 16384)                           */
+    int chunk_snps;         /* SNPs per chunk (0 = 16384; < 0: chosen by timing one
+                               chunk of 4,096, 8,192, 16,384 and 32,768 SNPs with
+                               the first requests, remembered per panel shape)     */
     int graphs;             /* 1: replay each chunk's launches from a HIP graph
                                captured once per active set and chunk size;
                                0: launch them one by one (the default)              */
@@ -75,6 +77,10 @@ const char *oagpu_server_last_error(const oagpu_server *server);
 
 /* Chunks this server has replayed from a captured graph so far. */
 long long oagpu_server_graph_chunks(const oagpu_server *server);
+
+/* SNPs per chunk: the configured size, or the timed choice once the first
+ * requests have been admitted (the largest candidate before that). */
+int oagpu_server_chunk_snps(const oagpu_server *server);
 
 /* Copies the request's inputs and queues it; returns at once.
  * OAGPU_ERR_UNSUPPORTED: a subset (keep with a zero flag) on a panel that

@@ -20,6 +20,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #include "../../include/oagpu_serve.h"
@@ -137,6 +138,11 @@ struct oagpu_server {
   long long g_seen[2] = {-1, -1};    // active set a slot last ran eagerly for
   bool graphs_off = false;           // replay not asked for, or a capture failed
   long long graph_chunks = 0;        // chunks replayed from a graph
+
+  // chunk size chosen by timing (chunk_snps < 0): buffers are sized for the
+  // largest candidate, the choice is made when the first requests arrive
+  bool tune_chunk = false;
+  int chunk_now = 0;                 // S->mc, readable under mu
 };
 
 #define SRVCK(S, expr)                                                                 \
@@ -464,6 +470,77 @@ int run_chunk(oagpu_server* S) {
   return OAGPU_OK;
 }
 
+// Chunk size by timing: one chunk of each candidate size (4,096 to 32,768
+// SNPs, at most the panel) with the first active set, after a run of it
+// that settles the contraction autotuner; the size with the least time per
+// SNP wins and is remembered per panel shape and precision.  The timing
+// runs write nothing the requests keep: their counters are cleared after,
+// and they start at chunk 0.
+std::mutex g_chunk_mu;
+std::map<std::tuple<int, int, long long, int, int>, int> g_chunk_choice;   // device, n, m, dtype, precision
+
+int tune_chunk_size(oagpu_server* S) {
+  const oagpu_panel* P = S->panel;
+  const auto key = std::make_tuple(P->device, P->n, P->m, P->dtype, S->prm.precision);
+  int best = 0;
+  {
+    std::lock_guard<std::mutex> lk(g_chunk_mu);
+    auto it = g_chunk_choice.find(key);
+    if (it != g_chunk_choice.end()) best = it->second;
+  }
+  if (best == 0) {
+    const int cap = S->mc;   // buffers hold this many SNPs
+    std::vector<int> cands;
+    for (int c : {4096, 8192, 16384, 32768})
+      if (c <= cap) cands.push_back(c);
+    if (cands.empty()) cands.push_back(cap);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    SRVCK(S, hipEventCreate(&e0));
+    SRVCK(S, hipEventCreate(&e1));
+    double best_per = 1e300;
+    int rc = OAGPU_OK;
+    for (int c : cands) {
+      const int mcc = (int)std::min<long long>(c, P->m);
+      if ((rc = enqueue_chunk(S, 0, mcc, nullptr))) break;   // settles the autotuner for this shape
+      if (hipEventRecord(e0, S->st) != hipSuccess) { rc = OAGPU_ERR_HIP; break; }
+      if ((rc = enqueue_chunk(S, 0, mcc, nullptr))) break;
+      if (hipEventRecord(e1, S->st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+        rc = OAGPU_ERR_HIP;
+        break;
+      }
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = OAGPU_ERR_HIP; break; }
+      const double per = ms / mcc;
+      if (per < best_per) {
+        best_per = per;
+        best = c;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) {
+      S->err = "chunk size tuning failed: " + S->err;
+      return rc;
+    }
+    std::lock_guard<std::mutex> lk(g_chunk_mu);
+    g_chunk_choice[key] = best;
+  }
+  // the timing runs' hits and counters go; every active request starts over
+  for (Request* R : S->active) {
+    launch_reset_counters(R->ctx->counters, CNT_N, S->st);
+    R->chunks_done = 0;
+  }
+  SRVCK(S, hipGetLastError());
+  SRVCK(S, hipStreamSynchronize(S->st));
+  std::lock_guard<std::mutex> lk(S->mu);
+  S->mc = best;
+  S->chunk_now = best;
+  S->nchunks = (int)((P->m + best - 1) / best);
+  S->cur = 0;
+  ++S->epoch;
+  return OAGPU_OK;
+}
+
 // Counters and hits of a request that has seen every chunk.
 void collect(oagpu_server* S, Request* R) {
   oagpu_ctx* c = R->ctx;
@@ -533,6 +610,10 @@ void server_loop(oagpu_server* S) {
     }
     if (S->active.empty()) continue;
     int rc = S->arena_dirty ? rebuild_arena(S) : OAGPU_OK;
+    if (!rc && S->tune_chunk) {
+      S->tune_chunk = false;
+      rc = tune_chunk_size(S);
+    }
     if (!rc) rc = run_chunk(S);
     if (rc) {   // a device error ends every active request
       for (Request* R : S->active) finish(S, R, rc, S->err);
@@ -576,9 +657,12 @@ extern "C" int oagpu_server_create(oagpu_panel* P, const oagpu_server_params* pr
   S->prm = *prm;
   if (S->prm.hbm_budget <= 0) S->prm.hbm_budget = 64ll << 30;
   if (S->prm.max_active <= 0) S->prm.max_active = 64;
-  if (S->prm.chunk_snps <= 0) S->prm.chunk_snps = 16384;
+  S->tune_chunk = S->prm.chunk_snps < 0;
+  if (S->prm.chunk_snps == 0) S->prm.chunk_snps = 16384;
+  if (S->tune_chunk) S->prm.chunk_snps = 32768;   // buffers for the largest candidate
   SRVCK(S, hipSetDevice(P->device));
   S->mc = (int)std::min<long long>(S->prm.chunk_snps, P->m);
+  S->chunk_now = S->mc;
   S->nchunks = (int)((P->m + S->mc - 1) / S->mc);
   S->op_size = prm->precision == OAGPU_FP64 ? 8 : (prm->precision == OAGPU_FP32 ? 4 : 2);
   S->c_size = prm->precision == OAGPU_FP64 ? 8 : 4;
@@ -638,6 +722,12 @@ extern "C" void oagpu_server_destroy(oagpu_server* S) {
 }
 
 extern "C" const char* oagpu_server_last_error(const oagpu_server* S) { return S ? S->err.c_str() : "null server"; }
+
+extern "C" int oagpu_server_chunk_snps(const oagpu_server* S) {
+  if (!S) return 0;
+  std::lock_guard<std::mutex> lk(const_cast<oagpu_server*>(S)->mu);
+  return S->chunk_now;
+}
 
 extern "C" long long oagpu_server_graph_chunks(const oagpu_server* S) {
   if (!S) return 0;

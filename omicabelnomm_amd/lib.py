@@ -89,7 +89,7 @@ EXPORTS = ["oagpu_create", "oagpu_destroy", "oagpu_last_error", "oagpu_set_subse
            "oagpu_scan", "oagpu_scan_async", "oagpu_scan_wait", "oagpu_host_alloc", "oagpu_host_free",
            "oagpu_last_timings", "oagpu_gemm_variant", "oagpu_gemm_tunings", "oagpu_tvalue",
            "oagpu_panel_create", "oagpu_panel_destroy", "oagpu_server_create", "oagpu_server_destroy",
-           "oagpu_server_last_error", "oagpu_server_graph_chunks", "oagpu_server_submit", "oagpu_server_poll", "oagpu_server_wait",
+           "oagpu_server_last_error", "oagpu_server_graph_chunks", "oagpu_server_chunk_snps", "oagpu_server_submit", "oagpu_server_poll", "oagpu_server_wait",
            "oagpu_scan_noalloc"]
 
 _lib = None
@@ -145,6 +145,8 @@ def load():
     L.oagpu_server_last_error.restype = ctypes.c_char_p
     L.oagpu_server_graph_chunks.argtypes = [vp]
     L.oagpu_server_graph_chunks.restype = ctypes.c_longlong
+    L.oagpu_server_chunk_snps.argtypes = [vp]
+    L.oagpu_server_chunk_snps.restype = ctypes.c_int
     L.oagpu_server_submit.argtypes = [vp, ctypes.POINTER(Request), ctypes.POINTER(ll)]
     L.oagpu_server_poll.argtypes = [vp, ll]
     L.oagpu_server_wait.argtypes = [vp, ll, vp, ll, ctypes.POINTER(Stats), ctypes.POINTER(ctypes.c_double),

@@ -81,6 +81,11 @@ class Server:
         self._max_hits[rid.value] = max_hits
         return rid.value
 
+    def chunk_snps(self) -> int:
+        """SNPs per chunk: the configured size, or (chunk_snps=-1) the size
+        chosen by timing once the first requests were admitted."""
+        return int(self._L.oagpu_server_chunk_snps(self._s))
+
     def graph_chunks(self) -> int:
         """Chunks replayed from a captured HIP graph so far."""
         return int(self._L.oagpu_server_graph_chunks(self._s))

@@ -109,6 +109,30 @@ def test_server_answers_equal_plain_scans(precision):
     panel.close()
 
 
+def test_chunk_size_chosen_by_timing():
+    """chunk_snps=-1: the server times one chunk of each candidate size with
+    the first requests and keeps the fastest per SNP; the timing runs leave
+    nothing in the answers, which equal plain scans of the whole panel."""
+    G = panel_data(n=520, m=40_000, seed=43)
+    n = G.shape[1]
+    reqs = [request_data(n, 12, 3, "groups", 11), request_data(n, 5, 4, "mcar", 12),
+            request_data(n, 20, 2, "none", 13)]
+    print("case chunk tuning", flush=True)
+    panel = Panel(G.to("cuda"))
+    with Server(panel, "fp64", chunk_snps=-1) as srv:
+        assert srv.chunk_snps() == 32768   # buffers for the largest candidate until the first requests arrive
+        ids = [srv.submit(Y, XL, store_all=True, **THR) for Y, XL in reqs]
+        answers = [srv.wait(i) for i in ids]
+        chosen = srv.chunk_snps()
+    panel.close()
+    print("chosen chunk", chosen, flush=True)
+    assert chosen in (4096, 8192, 16384, 32768)
+    for (Y, XL), a in zip(reqs, answers):
+        want = scan_alone(G, Y, XL, None, "fp64", **THR)
+        assert_hits_equal(a.hits, want, 1e-12)
+        assert a.stats["n_chunks"] == -(-40_000 // chosen)
+
+
 # split-BF16: the batch changes the contraction's shape, so the autotuner may
 # pick another kernel (another summation order); the bound is the one the
 # mixed path is held to against FP64
