@@ -76,6 +76,7 @@ struct Request {
   int status = OAGPU_OK;
   std::string err;
   int chunks_done = 0;
+  int overtaken = 0;   // requests admitted ahead of it while it did not fit
   int col_off = 0;
   long long bytes = 0;
   std::vector<oagpu_hit> hits;
@@ -84,6 +85,10 @@ struct Request {
 };
 
 }  // namespace
+
+// Times a request that does not fit the HBM budget may be passed by later
+// requests that do.
+constexpr int kMaxOvertaken = 8;
 
 // Streams the per-request kernels of a chunk are spread over (the server's
 // own stream and kLanes - 1 more): a request with few traits launches grids
@@ -587,14 +592,26 @@ void server_loop(oagpu_server* S) {
       std::unique_lock<std::mutex> lk(S->mu);
       S->cv.wait(lk, [&] { return S->stop || !S->queue.empty() || !S->active.empty(); });
       if (S->stop && S->active.empty()) break;
-      while (!S->stop && !S->queue.empty() && (int)(S->active.size() + admitted.size()) < S->prm.max_active) {
-        Request* R = S->queue.front();
+      // first fit in arrival order: a request that does not fit the
+      // budget beside the active ones lets later, smaller ones pass, at most
+      // kMaxOvertaken times; then nothing behind it is admitted until it fits
+      // (a request larger than the budget runs alone)
+      std::vector<Request*> skipped;
+      for (auto it = S->queue.begin();
+           !S->stop && it != S->queue.end() && (int)(S->active.size() + admitted.size()) < S->prm.max_active;) {
+        Request* R = *it;
         R->bytes = request_bytes(S, R);
         const bool empty = S->active.empty() && admitted.empty();
-        if (!empty && S->used_bytes + R->bytes > S->prm.hbm_budget) break;
-        S->queue.pop_front();
-        S->used_bytes += R->bytes;
-        admitted.push_back(R);
+        if (empty || S->used_bytes + R->bytes <= S->prm.hbm_budget) {
+          for (Request* W : skipped) ++W->overtaken;
+          it = S->queue.erase(it);
+          S->used_bytes += R->bytes;
+          admitted.push_back(R);
+          continue;
+        }
+        if (R->overtaken >= kMaxOvertaken) break;
+        skipped.push_back(R);
+        ++it;
       }
     }
     for (Request* R : admitted) {

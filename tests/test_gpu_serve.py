@@ -176,6 +176,31 @@ def test_budget_admits_one_at_a_time():
     panel.close()
 
 
+def test_small_request_passes_one_over_budget():
+    """Admission is first fit: while a long request runs, one that needs more
+    than the budget beside it waits, and a small one submitted after it is
+    admitted at once (it finishes first); the large one then runs alone."""
+    G = panel_data(n=300, m=20_000, seed=45)
+    n = G.shape[1]
+    long_req = request_data(n, 8, 3, "mcar", 30)
+    big_req = request_data(n, 6, 2, "mcar", 31)
+    small_req = request_data(n, 5, 2, "mcar", 32)
+    print("case first fit", flush=True)
+    panel = Panel(G.to("cuda"))
+    # the big request's hit arena (2^24 records of 56 bytes) exceeds the
+    # budget by itself; the others need a few MB
+    with Server(panel, "fp64", hbm_budget=600 << 20, chunk_snps=512) as srv:
+        a = srv.submit(*long_req, store_all=True, max_hits=1 << 18, **THR)
+        b = srv.submit(*big_req, store_all=True, max_hits=1 << 24, **THR)
+        c = srv.submit(*small_req, store_all=True, max_hits=1 << 18, **THR)
+        ans = {k: srv.wait(i) for k, i in (("a", a), ("b", b), ("c", c))}
+    for k, (Y, XL) in (("a", long_req), ("b", big_req), ("c", small_req)):
+        assert_hits_equal(ans[k].hits, scan_alone(G, Y, XL, None, "fp64", **THR), 1e-12)
+    print("latency ms", {k: round(v.latency_ms, 1) for k, v in ans.items()}, flush=True)
+    assert ans["c"].latency_ms < ans["b"].latency_ms
+    panel.close()
+
+
 @pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
 def test_panel_with_missing_genotypes(precision):
     """A panel holding missing genotypes (imputed with the panel-wide mean;
