@@ -81,11 +81,17 @@ __device__ __forceinline__ int sw(int r, int c) { return r * 32 + (((c ^ (r >> 1
 // The A stages hold the E*G rows of the tile's 128 SNPs in rows 0..127 and
 // the G rows in 128..255 (MFMA row 2s + t of the tile is stage row 128 t +
 // s), so the conversion's 8-byte stores of a wave fill consecutive rows.
-// Chunk c of stage row R sits at c xor (2 ((R >> 1) & 1) + (R >> 7)): the
-// 8 lanes an LDS cycle serves for an operand read (4 SNPs, both rows of
-// each) then hit 8 different 16-byte bank groups.
+// Chunk c of stage row R sits at c xor (2 ((R >> 2) & 1) + (R >> 7)).
+// gfx950 serves a 16-byte LDS read in groups of 16 lanes, 256 bytes over
+// the 64 banks per cycle: within each half wave, the lanes whose bits 2..4
+// have even parity, then those with odd parity (measured: two active lanes
+// reading one bank conflict exactly when they share a group).  For an
+// operand read, lane (i, g) reads chunk g of row R(i), and a group holds
+// i in {0-3, 12-15} with one g and i in {4-11} with the other: rows R and
+// R ^ 4 meet in one group with the same R & 3, so bit 2 of R flips the
+// chunk and the 16 lanes hit 16 different 16-byte bank groups.
 __device__ __forceinline__ int swa(int R, int c) {
-  return R * 32 + ((c ^ ((((R >> 1) & 1) << 1) | ((R >> 7) & 1))) << 3);
+  return R * 32 + ((c ^ ((((R >> 2) & 1) << 1) | ((R >> 7) & 1))) << 3);
 }
 
 // 16 bytes per lane from global memory into LDS at lds + 16 * lane.  From
