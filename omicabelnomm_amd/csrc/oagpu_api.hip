@@ -856,7 +856,7 @@ int oa::enqueue_chunks(oagpu_ctx* c, DevScan& d, long long mc, void* Ahi, void* 
                   c->st);
     };
     auto run_fused = [&]() {
-      launch_prep(d, PREC_SIDE, nullptr, nullptr, c->compact_corr, c->st);
+      launch_prep_pair(d, c->compact_corr, c->st);
       launch_gemm_fused(fused_args(c, d, M), c->gdt, c->st);
     };
     bool fused = false;
@@ -910,7 +910,7 @@ int oa::enqueue_chunks(oagpu_ctx* c, DevScan& d, long long mc, void* Ahi, void* 
     c->last_fused = fused;
     if (Ec) HIPCK(c, hipEventRecord(Ec[0], c->st));
     if (fused) {
-      launch_prep(d, PREC_SIDE, nullptr, nullptr, c->compact_corr, c->st);
+      launch_prep_pair(d, c->compact_corr, c->st);
       if (Ec) HIPCK(c, hipEventRecord(Ec[1], c->st));
       launch_gemm_fused(fused_args(c, d, M), c->gdt, c->st);
     } else {

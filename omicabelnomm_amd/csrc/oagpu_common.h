@@ -6,9 +6,6 @@
 namespace oa {
 
 enum { PREC_FP64 = 0, PREC_FP32 = 1, PREC_BF16X3 = 2 };
-// operand pass without the operand: means, Gram pieces and compact rows only
-// (the fused contraction forms its operand from the genotypes itself)
-enum { PREC_SIDE = 3 };
 enum { DT_F64 = 0, DT_F32 = 1, DT_F16 = 2 };
 
 // Largest coefficient count a pair can have (l + r), and the stride of a

@@ -31,6 +31,9 @@ void launch_trait_solve(const DevScan& d, int lb, double* zt, double* ay, double
 // corr: also write the imputed values at the rows of U (d.Xu), read by
 // launch_corr.
 void launch_prep(const DevScan& d, int prec, void* Ahi, void* Alo, bool corr, hipStream_t st);
+// the operand pass below the fused contraction ([E*G, G] pairs, no subset):
+// means, missing flags, Sbr and compact rows, no operand
+void launch_prep_pair(const DevScan& d, bool corr, hipStream_t st);
 // Xu[u][s] = G[(s0 + s) * ldg + u] for u < n, s < mc (gdt DT_F32 or DT_F16)
 void launch_panel_rows(const void* G, int gdt, long long ldg, int s0, int mc, int n, void* Xu, long long ldxu,
                        hipStream_t st, const int* s0_dev = nullptr);

@@ -18,9 +18,7 @@ constexpr int prep_spw() { return R == 1 ? 4 : (R == 2 ? 2 : 1); }
 
 template <int OP>
 __device__ __forceinline__ void store_op4(void* Ahi, void* Alo, size_t o, const double* v) {
-  if constexpr (OP == PREC_SIDE) {
-    return;
-  } else if (OP == PREC_FP64) {
+  if constexpr (OP == PREC_FP64) {
     d4 x = {v[0], v[1], v[2], v[3]};
     *reinterpret_cast<d4*>(reinterpret_cast<double*>(Ahi) + o) = x;
   } else if (OP == PREC_FP32) {
@@ -272,6 +270,204 @@ __global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo) {
 }
 
 // ---------------------------------------------------------------------------
+// The operand pass below the fused contraction, for blocks of [E*G, G]
+// pairs (every SNP: one genotype column G, output columns E*G and G, one
+// interaction row E; no subset).  No operand is written, so the pass is a
+// read of the genotypes: column means and missing flags, Sbr and the
+// compact rows of U.  One wave owns SPW_P SNPs and walks their columns
+// alone (the sums close with shuffles, no barrier); a genotype is loaded
+// once for both of its columns, and each value costs 7 FP64 operations:
+// the conversion, E*G, two sums, three Gram terms.  A wave whose sums come
+// out NaN repeats its walk: counting the missing values, then with the
+// means imputed (replace_nans_avgs, src/Utility.cpp:103), overwriting
+// what the first walk wrote.
+constexpr int SPW_P = 4;   // SNPs per wave
+static_assert(SPW_P == 4, "the compact-row stores below write four SNPs");
+
+template <typename XT, typename GT>
+__global__ __launch_bounds__(256) void k_prep_pair(DevScan d) {
+  chunk_start(d);
+  constexpr bool CORR = !std::is_void<XT>::value;
+  typedef typename GVec<GT>::T GV;
+  const int lane = threadIdx.x & 63;
+  // blocks dealt to the XCDs in turn: consecutive blocks on one XCD, so the
+  // pieces of neighbouring SNPs in one compact-row line meet in one L2
+  const int per = (gridDim.x + 7) >> 3;
+  const int blk = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  const int sb = (blk * 4 + (threadIdx.x >> 6)) * SPW_P;   // the wave's first SNP
+  if (sb >= d.mc) return;   // wave-uniform; the kernel has no barrier
+  bool live[SPW_P];
+  int src[SPW_P], env = -1;
+#pragma unroll
+  for (int q = 0; q < SPW_P; ++q) {
+    live[q] = sb + q < d.mc;
+    int s2[2], e2[2];
+    col_desc<2>(d, d.s0 + (live[q] ? sb + q : sb), s2, e2);
+    src[q] = s2[0];
+    env = e2[0];   // the block's one interaction row
+  }
+  const double* Erow = d.E + (size_t)env * d.lde;   // padded to n_pad with zeros
+  // per SNP: sums of E*G and G, missing counts, Gram terms (EG EG, EG G, G G)
+  double sum[SPW_P][2], cnt[SPW_P][2], mean[SPW_P][2], gram[SPW_P][3];
+#pragma unroll
+  for (int q = 0; q < SPW_P; ++q)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) { sum[q][c] = 0.0; cnt[q][c] = 0.0; mean[q][c] = 0.0; }
+  // IMPUTE (0: unchecked sums, 1: count the missing values, 2: impute)
+  auto walk = [&](auto MODE) {
+    constexpr int mode = decltype(MODE)::value;
+#pragma unroll
+    for (int q = 0; q < SPW_P; ++q)
+#pragma unroll
+      for (int e = 0; e < 3; ++e) gram[q][e] = 0.0;
+    // two runs in flight per lane: a run's loads are issued two runs ahead
+    struct Buf { GV g[SPW_P]; d4 e; int4 u; };
+    auto fetch = [&](int k, Buf& b) {
+      const bool vec = d.g_vec && __builtin_amdgcn_readfirstlane(k - 4 * lane) + 256 <= d.n;
+#pragma unroll
+      for (int q = 0; q < SPW_P; ++q) b.g[q] = load_g4<GT>(d, src[q], k, vec);
+      b.e = *reinterpret_cast<const d4*>(Erow + k);
+      if constexpr (CORR) b.u = *reinterpret_cast<const int4*>(d.uidx + k);
+    };
+    auto body = [&](int k0, Buf& b) {
+      GV gv[SPW_P];
+#pragma unroll
+      for (int q = 0; q < SPW_P; ++q) gv[q] = b.g[q];
+      const d4 ev = b.e;
+      const int4 ui = b.u;
+      if (k0 + 512 < d.n_pad) fetch(k0 + 512, b);
+      double x[SPW_P][2][4];
+#pragma unroll
+      for (int q = 0; q < SPW_P; ++q)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          double g;
+          if constexpr (std::is_same<GT, _Float16>::value) g = (double)(float)gv[q][e];
+          else g = (double)gv[q][e];
+          double eg = g * ev[e];
+          if constexpr (mode == 0) {
+            sum[q][0] += eg;
+            sum[q][1] += g;
+          } else if constexpr (mode == 1) {
+            if (eg == eg) sum[q][0] += eg; else cnt[q][0] += 1.0;
+            if (g == g) sum[q][1] += g; else cnt[q][1] += 1.0;
+          } else {
+            if (eg != eg) eg = mean[q][0];
+            if (g != g) g = mean[q][1];
+          }
+          if constexpr (mode != 1) {
+            gram[q][0] += eg * eg;
+            gram[q][1] += eg * g;
+            gram[q][2] += g * g;
+          }
+          x[q][0][e] = eg;
+          x[q][1][e] = g;
+        }
+      if constexpr (CORR) {
+        if constexpr (mode != 1) {
+          // rows of U: the SNPs' values, u-major (EG, G of each SNP); a
+          // block's 16 SNPs fill one 128-byte line of a float row
+          const int uu[4] = {ui.x, ui.y, ui.z, ui.w};
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            if (uu[e] >= 0) {
+              XT* dst = static_cast<XT*>(d.Xu) + (size_t)uu[e] * d.ldxu + (size_t)sb * 2;
+              if (live[SPW_P - 1]) {
+                // vector stores: 2 sb and ldxu are multiples of 4
+                const double v0[4] = {x[0][0][e], x[0][1][e], x[1][0][e], x[1][1][e]};
+                const double v1[4] = {x[2][0][e], x[2][1][e], x[3][0][e], x[3][1][e]};
+                store_xu4(dst, v0);
+                store_xu4(dst + 4, v1);
+              } else {
+#pragma unroll
+                for (int q = 0; q < SPW_P; ++q)
+                  if (live[q]) {
+                    store_xu(dst + 2 * q, x[q][0][e]);
+                    store_xu(dst + 2 * q + 1, x[q][1][e]);
+                  }
+              }
+            }
+        }
+      }
+    };
+    Buf b0, b1;
+    b0.u = b1.u = make_int4(-1, -1, -1, -1);
+    const int k = 4 * lane;
+    if (k < d.n_pad) fetch(k, b0);
+    if (k + 256 < d.n_pad) fetch(k + 256, b1);
+    for (int k0 = k; k0 < d.n_pad; k0 += 512) {
+      body(k0, b0);
+      if (k0 + 256 < d.n_pad) body(k0 + 256, b1);
+    }
+  };
+  auto close = [&]() {
+#pragma unroll
+    for (int q = 0; q < SPW_P; ++q) {
+#pragma unroll
+      for (int c = 0; c < 2; ++c) { sum[q][c] = wave_sum(sum[q][c]); cnt[q][c] = wave_sum(cnt[q][c]); }
+#pragma unroll
+      for (int e = 0; e < 3; ++e) gram[q][e] = wave_sum(gram[q][e]);
+    }
+  };
+  walk(std::integral_constant<int, 0>{});
+  close();
+  bool nan_sum = false;
+#pragma unroll
+  for (int q = 0; q < SPW_P; ++q) nan_sum = nan_sum || (live[q] && (sum[q][0] != sum[q][0] || sum[q][1] != sum[q][1]));
+  if (nan_sum) {   // wave-uniform: count the missing values, sum the others, then impute
+#pragma unroll
+    for (int q = 0; q < SPW_P; ++q)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) sum[q][c] = 0.0;
+    walk(std::integral_constant<int, 1>{});
+    close();
+#pragma unroll
+    for (int q = 0; q < SPW_P; ++q)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) mean[q][c] = sum[q][c] / ((double)d.n_stat - cnt[q][c]);
+    walk(std::integral_constant<int, 2>{});
+#pragma unroll
+    for (int q = 0; q < SPW_P; ++q)
+#pragma unroll
+      for (int e = 0; e < 3; ++e) gram[q][e] = wave_sum(gram[q][e]);
+  } else {
+#pragma unroll
+    for (int q = 0; q < SPW_P; ++q)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) mean[q][c] = sum[q][c] / (double)d.n_stat;
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < SPW_P; ++q)
+      if (live[q]) {
+        const int s = sb + q;
+#pragma unroll
+        for (int e = 0; e < 3; ++e) d.Sbr[(size_t)s * 3 + e] = gram[q][e];   // upk: (0,0), (0,1), (1,1)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          d.col_mean[(size_t)s * 2 + c] = mean[q][c];
+          if (d.col_nan) d.col_nan[(size_t)s * 2 + c] = cnt[q][c] > 0.0 ? 1 : 0;
+        }
+      }
+  }
+}
+
+template <typename XT>
+static void prep_pair_x(const DevScan& d, hipStream_t st) {
+  const int waves = (d.mc + SPW_P - 1) / SPW_P;
+  const int nb = (waves + 3) / 4;
+  dim3 grid((nb + 7) / 8 * 8), block(256);   // a multiple of 8 for the XCD mapping
+  if (d.gdt == DT_F32) hipLaunchKernelGGL((k_prep_pair<XT, float>), grid, block, 0, st, d);
+  else hipLaunchKernelGGL((k_prep_pair<XT, _Float16>), grid, block, 0, st, d);
+}
+
+void launch_prep_pair(const DevScan& d, bool corr, hipStream_t st) {
+  if (!corr) prep_pair_x<void>(d, st);
+  else if (d.xu_dt == DT_F64) prep_pair_x<double>(d, st);
+  else prep_pair_x<float>(d, st);
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 template <int R, typename XT, typename GT>
 static void prep_rxg(const DevScan& d, int prec, void* Ahi, void* Alo, hipStream_t st) {
@@ -279,7 +475,6 @@ static void prep_rxg(const DevScan& d, int prec, void* Ahi, void* Alo, hipStream
   dim3 grid((nb + 7) / 8 * 8), block(256);   // a multiple of 8 for the XCD mapping
   if (prec == PREC_FP64) hipLaunchKernelGGL((k_prep<R, PREC_FP64, XT, GT>), grid, block, 0, st, d, Ahi, Alo);
   else if (prec == PREC_FP32) hipLaunchKernelGGL((k_prep<R, PREC_FP32, XT, GT>), grid, block, 0, st, d, Ahi, Alo);
-  else if (prec == PREC_SIDE) hipLaunchKernelGGL((k_prep<R, PREC_SIDE, XT, GT>), grid, block, 0, st, d, Ahi, Alo);
   else hipLaunchKernelGGL((k_prep<R, PREC_BF16X3, XT, GT>), grid, block, 0, st, d, Ahi, Alo);
 }
 
@@ -289,7 +484,6 @@ static void prep_rxgo(const DevScan& d, int prec, void* Ahi, void* Alo, hipStrea
   const int nb = (d.mc + prep_spw<R>() - 1) / prep_spw<R>();
   dim3 grid((nb + 7) / 8 * 8), block(256);
   if (prec == PREC_FP32) hipLaunchKernelGGL((k_prep<R, PREC_FP32, XT, GT>), grid, block, 0, st, d, Ahi, Alo);
-  else if (prec == PREC_SIDE) hipLaunchKernelGGL((k_prep<R, PREC_SIDE, XT, GT>), grid, block, 0, st, d, Ahi, Alo);
   else hipLaunchKernelGGL((k_prep<R, PREC_BF16X3, XT, GT>), grid, block, 0, st, d, Ahi, Alo);
 }
 
