@@ -786,7 +786,10 @@ static int ensure_chunk(oagpu_ctx* c, int mc) {
   const size_t arows = ((size_t)mc * r + pad - 1) / pad * pad;
   const size_t abytes = arows * c->ldop * c->op_size;
   const size_t cbytes = (size_t)mc * r * out_cols(c) * c->c_size;
-  c->ldxu = ((long long)mc * r + 7) / 8 * 8;
+  // 32 elements of padding: the corrections read 512-byte pieces of many
+  // rows at once, and a row stride off the chunk size's round multiple
+  // measured 1.5 % faster at n = 5,000 (100,000 SNPs per chunk)
+  c->ldxu = ((long long)mc * r + 7) / 8 * 8 + 32;
   // rows 0..nU-1 of U, then a zero row (nU) for the corrections' padding lanes
   const size_t xubytes = c->compact_corr ? xu_size(c) * ((size_t)c->nU + 1) * c->ldxu : 0;
   int rc = ensure_pair_buffers(c, mc);

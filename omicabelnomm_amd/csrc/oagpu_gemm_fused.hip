@@ -27,12 +27,10 @@
 // stage rows are those 128 bytes with the 16-byte chunks XOR-swizzled by
 // row & 7.  A's hi and lo halves are separate arrays of 64-byte rows, the
 // tile's E*G rows in rows 0..127 and its G rows in 128..255 with their own
-// swizzle (swa), so a wave's conversion stores fill consecutive rows.
-// SQ_LDS_BANK_CONFLICT reads 2.0e9 cycles per n = 10,000 launch with this
-// kernel whatever the A layout (interleaved rows with the original wave
-// order: 1.0e9; a layout that puts a SNP's two rows in different bank
-// halves: again 2.0e9, and slower); a padded 80-byte row measured 1.33x
-// slower.  Per step s:
+// swizzle (swa), so a wave's conversion stores fill consecutive rows; no
+// LDS access of the kernel has a bank conflict (SQ_LDS_BANK_CONFLICT reads
+// 0 per n = 10,000 launch; a padded 80-byte row measured 1.33x slower).
+// Per step s:
 //   1. convert the genotypes of step s+1 (loaded two steps earlier) into the
 //      other stage;
 //   2. DMA B's k-slice of step s+1 into the other stage (global_load_lds,
@@ -74,9 +72,6 @@ __device__ __forceinline__ int xcd_tile_f(int id, int total) {
   const int xcd = id & 7, loc = id >> 3;
   return xcd * q + (xcd < rem ? xcd : rem) + loc;
 }
-
-// bf16 offset of 16-byte chunk c of row r (64-byte rows, chunk xor (r >> 1))
-__device__ __forceinline__ int sw(int r, int c) { return r * 32 + (((c ^ (r >> 1)) & 3) << 3); }
 
 // The A stages hold the E*G rows of the tile's 128 SNPs in rows 0..127 and
 // the G rows in 128..255 (MFMA row 2s + t of the tile is stage row 128 t +
