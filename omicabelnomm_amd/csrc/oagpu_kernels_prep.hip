@@ -274,69 +274,94 @@ __global__ __launch_bounds__(256) void k_prep(DevScan d, void* Ahi, void* Alo) {
 // pairs (every SNP: one genotype column G, output columns E*G and G, one
 // interaction row E; no subset).  No operand is written, so the pass is a
 // read of the genotypes: column means and missing flags, Sbr and the
-// compact rows of U.  One wave owns SPW_P SNPs and walks their columns
-// alone (the sums close with shuffles, no barrier); a genotype is loaded
-// once for both of its columns, and each value costs 7 FP64 operations:
-// the conversion, E*G, two sums, three Gram terms.  A wave whose sums come
-// out NaN repeats its walk: counting the missing values, then with the
-// means imputed (replace_nans_avgs, src/Utility.cpp:103), overwriting
-// what the first walk wrote.
+// compact rows of U.  A block owns 16 SNPs, one wave 4 of them; a genotype
+// is loaded once for both of its columns, and each value costs 7 FP64
+// operations: the conversion, E*G, two sums, three Gram terms.  The four
+// waves walk the rows in step, 256 at a time: the values at that range's
+// rows of U go to LDS, and after a barrier the block writes each such row's
+// 16 SNPs as one 128-byte line (one store instruction per 32 rows of U,
+// where per-lane stores took one per row position with few lanes active).
+// A block whose sums come out NaN repeats its walk: counting the missing
+// values, then with the means imputed (replace_nans_avgs,
+// src/Utility.cpp:103), overwriting what the first walk wrote.
 constexpr int SPW_P = 4;   // SNPs per wave
-static_assert(SPW_P == 4, "the compact-row stores below write four SNPs");
+static_assert(SPW_P == 4, "the compact-row staging below holds four SNPs per wave");
 
-template <typename XT, typename GT>
-__global__ __launch_bounds__(256) void k_prep_pair(DevScan d) {
+// CORR: write the compact rows (FP32, the type below the split-BF16
+// contraction).
+template <bool CORR, typename GT>
+__global__ __launch_bounds__(256, 2) void k_prep_pair(DevScan d) {
   chunk_start(d);
-  constexpr bool CORR = !std::is_void<XT>::value;
   typedef typename GVec<GT>::T GV;
-  const int lane = threadIdx.x & 63;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   // blocks dealt to the XCDs in turn: consecutive blocks on one XCD, so the
-  // pieces of neighbouring SNPs in one compact-row line meet in one L2
+  // lines of neighbouring blocks in one row of U meet in one L2
   const int per = (gridDim.x + 7) >> 3;
   const int blk = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-  const int sb = (blk * 4 + (threadIdx.x >> 6)) * SPW_P;   // the wave's first SNP
-  if (sb >= d.mc) return;   // wave-uniform; the kernel has no barrier
+  const int s_blk = blk * 16;                 // the block's first SNP
+  if (s_blk >= d.mc) return;                  // block-uniform, before any barrier
+  const int sb = s_blk + w * SPW_P;           // the wave's first SNP
   bool live[SPW_P];
   int src[SPW_P], env = -1;
 #pragma unroll
   for (int q = 0; q < SPW_P; ++q) {
     live[q] = sb + q < d.mc;
     int s2[2], e2[2];
-    col_desc<2>(d, d.s0 + (live[q] ? sb + q : sb), s2, e2);
+    col_desc<2>(d, d.s0 + min(sb + q, d.mc - 1), s2, e2);
     src[q] = s2[0];
     env = e2[0];   // the block's one interaction row
   }
   const double* Erow = d.E + (size_t)env * d.lde;   // padded to n_pad with zeros
+  // the range's rows of U, 16 SNPs x (E*G, G) each, two buffers
+  __shared__ __attribute__((aligned(16))) float xs[CORR ? 2 : 1][CORR ? 256 : 1][32];
   // per SNP: sums of E*G and G, missing counts, Gram terms (EG EG, EG G, G G)
   double sum[SPW_P][2], cnt[SPW_P][2], mean[SPW_P][2], gram[SPW_P][3];
 #pragma unroll
   for (int q = 0; q < SPW_P; ++q)
 #pragma unroll
     for (int c = 0; c < 2; ++c) { sum[q][c] = 0.0; cnt[q][c] = 0.0; mean[q][c] = 0.0; }
-  // IMPUTE (0: unchecked sums, 1: count the missing values, 2: impute)
+  // MODE 0: unchecked sums, 1: count the missing values, 2: impute
   auto walk = [&](auto MODE) {
     constexpr int mode = decltype(MODE)::value;
+    constexpr bool STORE = CORR && mode != 1;
 #pragma unroll
     for (int q = 0; q < SPW_P; ++q)
 #pragma unroll
       for (int e = 0; e < 3; ++e) gram[q][e] = 0.0;
-    // two runs in flight per lane: a run's loads are issued two runs ahead
+    // two ranges in flight per lane: a range's loads are issued two ahead
     struct Buf { GV g[SPW_P]; d4 e; int4 u; };
     auto fetch = [&](int k, Buf& b) {
+      if (k >= d.n_pad) return;
       const bool vec = d.g_vec && __builtin_amdgcn_readfirstlane(k - 4 * lane) + 256 <= d.n;
 #pragma unroll
       for (int q = 0; q < SPW_P; ++q) b.g[q] = load_g4<GT>(d, src[q], k, vec);
       b.e = *reinterpret_cast<const d4*>(Erow + k);
       if constexpr (CORR) b.u = *reinterpret_cast<const int4*>(d.uidx + k);
     };
-    auto body = [&](int k0, Buf& b) {
+    // the range of rows base .. base + 255 (every wave and lane takes the
+    // same path: the loop and its barriers are block-uniform)
+    auto body = [&](int base, Buf& b, int buf) {
+      const int k0 = base + 4 * lane;
+      const bool in = k0 < d.n_pad;
       GV gv[SPW_P];
 #pragma unroll
       for (int q = 0; q < SPW_P; ++q) gv[q] = b.g[q];
       const d4 ev = b.e;
-      const int4 ui = b.u;
-      if (k0 + 512 < d.n_pad) fetch(k0 + 512, b);
-      double x[SPW_P][2][4];
+      const int4 ui = in ? b.u : make_int4(-1, -1, -1, -1);
+      fetch(k0 + 512, b);
+      // the range's rows of U: the first one's position and their count (U
+      // is sorted, so the range holds positions umin .. umin + nr - 1)
+      const int uu[4] = {ui.x, ui.y, ui.z, ui.w};
+      int umin = 0x7fffffff, nr = 0;
+      if constexpr (STORE) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          if (uu[e] >= 0) umin = min(umin, uu[e]);
+          nr += __builtin_popcountll(__ballot(uu[e] >= 0));
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) umin = min(umin, __shfl_xor(umin, off, 64));
+      }
 #pragma unroll
       for (int q = 0; q < SPW_P; ++q)
 #pragma unroll
@@ -344,7 +369,9 @@ __global__ __launch_bounds__(256) void k_prep_pair(DevScan d) {
           double g;
           if constexpr (std::is_same<GT, _Float16>::value) g = (double)(float)gv[q][e];
           else g = (double)gv[q][e];
+          if (!in) g = 0.0;
           double eg = g * ev[e];
+          if (!in) eg = 0.0;
           if constexpr (mode == 0) {
             sum[q][0] += eg;
             sum[q][1] += g;
@@ -360,44 +387,35 @@ __global__ __launch_bounds__(256) void k_prep_pair(DevScan d) {
             gram[q][1] += eg * g;
             gram[q][2] += g * g;
           }
-          x[q][0][e] = eg;
-          x[q][1][e] = g;
+          // the compact rows keep FP32 values
+          if constexpr (STORE)
+            if (uu[e] >= 0)
+              *reinterpret_cast<float2*>(&xs[buf][uu[e] - umin][8 * w + 2 * q]) = make_float2((float)eg, (float)g);
         }
-      if constexpr (CORR) {
-        if constexpr (mode != 1) {
-          // rows of U: the SNPs' values, u-major (EG, G of each SNP); a
-          // block's 16 SNPs fill one 128-byte line of a float row
-          const int uu[4] = {ui.x, ui.y, ui.z, ui.w};
-#pragma unroll
-          for (int e = 0; e < 4; ++e)
-            if (uu[e] >= 0) {
-              XT* dst = static_cast<XT*>(d.Xu) + (size_t)uu[e] * d.ldxu + (size_t)sb * 2;
-              if (live[SPW_P - 1]) {
-                // vector stores: 2 sb and ldxu are multiples of 4
-                const double v0[4] = {x[0][0][e], x[0][1][e], x[1][0][e], x[1][1][e]};
-                const double v1[4] = {x[2][0][e], x[2][1][e], x[3][0][e], x[3][1][e]};
-                store_xu4(dst, v0);
-                store_xu4(dst + 4, v1);
-              } else {
-#pragma unroll
-                for (int q = 0; q < SPW_P; ++q)
-                  if (live[q]) {
-                    store_xu(dst + 2 * q, x[q][0][e]);
-                    store_xu(dst + 2 * q + 1, x[q][1][e]);
-                  }
-              }
-            }
+      if constexpr (CORR) if constexpr (STORE) {
+        if (nr > 0) {   // block-uniform: every wave sees the same rows
+          __syncthreads();
+          // 8 threads per row of U: columns 2 s_blk + 4 c .. + 3, those of
+          // SNPs past the chunk left alone
+          const int c = tid & 7, col = 2 * s_blk + 4 * c;
+          for (int r = tid >> 3; r < nr; r += 32) {
+            const float4 v = *reinterpret_cast<const float4*>(xs[buf][r] + 4 * c);
+            float* dst = static_cast<float*>(d.Xu) + (size_t)(umin + r) * d.ldxu + col;
+            if (col + 4 <= 2 * d.mc) *reinterpret_cast<float4*>(dst) = v;
+            else if (col < 2 * d.mc) *reinterpret_cast<float2*>(dst) = make_float2(v.x, v.y);   // 2 d.mc is even
+          }
         }
       }
     };
     Buf b0, b1;
     b0.u = b1.u = make_int4(-1, -1, -1, -1);
-    const int k = 4 * lane;
-    if (k < d.n_pad) fetch(k, b0);
-    if (k + 256 < d.n_pad) fetch(k + 256, b1);
-    for (int k0 = k; k0 < d.n_pad; k0 += 512) {
-      body(k0, b0);
-      if (k0 + 256 < d.n_pad) body(k0 + 256, b1);
+    fetch(4 * lane, b0);
+    fetch(4 * lane + 256, b1);
+    // two buffers of the staged rows: a buffer is refilled two ranges later,
+    // after the next range's barrier, by which time every thread has stored it
+    for (int base = 0; base < d.n_pad; base += 512) {
+      body(base, b0, 0);
+      if (base + 256 < d.n_pad) body(base + 256, b1, CORR ? 1 : 0);
     }
   };
   auto close = [&]() {
@@ -414,7 +432,7 @@ __global__ __launch_bounds__(256) void k_prep_pair(DevScan d) {
   bool nan_sum = false;
 #pragma unroll
   for (int q = 0; q < SPW_P; ++q) nan_sum = nan_sum || (live[q] && (sum[q][0] != sum[q][0] || sum[q][1] != sum[q][1]));
-  if (nan_sum) {   // wave-uniform: count the missing values, sum the others, then impute
+  if (__syncthreads_or(nan_sum)) {   // block-uniform: count the missing values, sum the others, then impute
 #pragma unroll
     for (int q = 0; q < SPW_P; ++q)
 #pragma unroll
@@ -452,19 +470,19 @@ __global__ __launch_bounds__(256) void k_prep_pair(DevScan d) {
   }
 }
 
-template <typename XT>
+template <bool CORR>
 static void prep_pair_x(const DevScan& d, hipStream_t st) {
-  const int waves = (d.mc + SPW_P - 1) / SPW_P;
-  const int nb = (waves + 3) / 4;
+  const int nb = (d.mc + 15) / 16;
   dim3 grid((nb + 7) / 8 * 8), block(256);   // a multiple of 8 for the XCD mapping
-  if (d.gdt == DT_F32) hipLaunchKernelGGL((k_prep_pair<XT, float>), grid, block, 0, st, d);
-  else hipLaunchKernelGGL((k_prep_pair<XT, _Float16>), grid, block, 0, st, d);
+  if (d.gdt == DT_F32) hipLaunchKernelGGL((k_prep_pair<CORR, float>), grid, block, 0, st, d);
+  else hipLaunchKernelGGL((k_prep_pair<CORR, _Float16>), grid, block, 0, st, d);
 }
 
 void launch_prep_pair(const DevScan& d, bool corr, hipStream_t st) {
-  if (!corr) prep_pair_x<void>(d, st);
-  else if (d.xu_dt == DT_F64) prep_pair_x<double>(d, st);
-  else prep_pair_x<float>(d, st);
+  // the compact rows are FP32 below the split-BF16 contraction, the only
+  // one this pass serves
+  if (corr) prep_pair_x<true>(d, st);
+  else prep_pair_x<false>(d, st);
 }
 
 // ---------------------------------------------------------------------------
