@@ -90,6 +90,9 @@ int oa_oracle_exclusions(const char *text, int n, uint8_t *keep);
  * (widened to double).  The CPU baseline of the benchmark. */
 int oa_ref32_scan(const oa_oracle_params *prm, const float *XL, const float *XR, const float *Y,
                   int tb, int mb, oa_oracle_out *out);
+/* The BLAS the FP32 restatement's products run through (MKL loaded at run
+ * time, or the file's own kernel). */
+const char *oa_ref32_blas(void);
 
 #ifdef __cplusplus
 }

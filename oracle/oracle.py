@@ -69,8 +69,15 @@ def lib():
         L.oa_ref32_scan.argtypes = [ctypes.POINTER(_Params), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                     ctypes.c_int, ctypes.c_int, ctypes.POINTER(_Out)]
         L.oa_ref32_scan.restype = ctypes.c_int
+        L.oa_ref32_blas.argtypes = []
+        L.oa_ref32_blas.restype = ctypes.c_char_p
         _lib = L
     return _lib
+
+
+def ref32_blas() -> str:
+    """The BLAS behind the FP32 restatement's products."""
+    return lib().oa_ref32_blas().decode()
 
 
 def tvalue(pval: float) -> float:

@@ -26,8 +26,11 @@
  *       sum in long double (singleSSY) .. src/Algorithm.cpp:976-1024
  *       hpc_statistics ................... src/Algorithm.cpp:1034-1214
  * Every value is a float, as in the reference, except the RSS sum.  The
- * BLAS products run through a cache-blocked FP32 kernel of this file (GCC
- * vector extensions, OpenMP); the image has no CBLAS.
+ * BLAS products run through the image's MKL (cblas_sgemm of
+ * /opt/conda/lib/libmkl_rt.so.1, loaded at run time, GNU OpenMP threading),
+ * or, where it cannot be loaded or OA_REF32_BLAS=builtin is set, through a
+ * cache-blocked FP32 kernel of this file (GCC vector extensions, OpenMP);
+ * oa_ref32_blas() names the one in use.
  */
 #include "oracle.h"
 
@@ -37,7 +40,9 @@
 #include <cstring>
 #include <vector>
 
+#include <dlfcn.h>
 #include <omp.h>
+#include <cstdlib>
 
 namespace {
 
@@ -49,12 +54,50 @@ inline float hsum(f8 v) {
     return s;
 }
 
+/* A run-time CBLAS: MKL's cblas_sgemm, loaded once. */
+typedef void (*sgemm_fn)(int, int, int, int, int, int, float, const float *, int, const float *, int, float,
+                         float *, int);
+struct Blas {
+    sgemm_fn sgemm = nullptr;
+    void (*set_threads)(int) = nullptr;
+    const char *name = "FP32 cache-blocked kernel of oracle/ref32.cpp (AVX2/FMA vectors, OpenMP)";
+};
+const Blas &blas() {
+    static const Blas b = [] {
+        Blas r;
+        const char *force = std::getenv("OA_REF32_BLAS");
+        if (force && !std::strcmp(force, "builtin")) return r;
+        for (const char *path : {"/opt/conda/lib/libmkl_rt.so.1", "libmkl_rt.so.1", "libmkl_rt.so"}) {
+            void *h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+            if (!h) continue;
+            auto layer = reinterpret_cast<int (*)(int)>(dlsym(h, "MKL_Set_Threading_Layer"));
+            auto gemm = reinterpret_cast<sgemm_fn>(dlsym(h, "cblas_sgemm"));
+            if (!gemm) continue;
+            if (layer) layer(3);   // MKL_THREADING_GNU: the same OpenMP runtime as this file
+            r.sgemm = gemm;
+            r.set_threads = reinterpret_cast<void (*)(int)>(dlsym(h, "MKL_Set_Num_Threads"));
+            r.name = "MKL cblas_sgemm (/opt/conda/lib/libmkl_rt.so.1, GNU OpenMP threads)";
+            break;
+        }
+        return r;
+    }();
+    return b;
+}
+
 /* C[i + j*ldc] = sum_k A[k + i*lda] B[k + j*ldb], i < M, j < N, k < K:
  * A'B for column-major A (K x M) and B (K x N), the shape of every product
- * of the scan (columns of length n).  4 x 4 blocks of dot products, eight
- * k-values per vector; OpenMP over the blocks. */
+ * of the scan (columns of length n).  With a CBLAS: one sgemm (A transposed,
+ * B not).  Else 4 x 4 blocks of dot products, eight k-values per vector;
+ * OpenMP over the blocks. */
 void gemm_tn(int M, int N, int K, const float *A, long lda, const float *B, long ldb, float *C, long ldc,
              int threads) {
+    const Blas &b = blas();
+    if (b.sgemm) {
+        if (b.set_threads) b.set_threads(threads);
+        b.sgemm(102 /* column major */, 112 /* A transposed */, 111 /* B not */, M, N, K, 1.0f, A, (int)lda, B,
+                (int)ldb, 0.0f, C, (int)ldc);
+        return;
+    }
     const int K8 = K / 8 * 8;
     const int mb = (M + 3) / 4, nb = (N + 3) / 4;
 #pragma omp parallel for schedule(dynamic, 4) num_threads(threads) collapse(2)
@@ -151,6 +194,8 @@ double pvalue(double t) {
 }
 
 }  // namespace
+
+extern "C" const char *oa_ref32_blas() { return blas().name; }
 
 extern "C" int oa_ref32_scan(const oa_oracle_params *prm, const float *XL, const float *XR, const float *Y,
                              int tb, int mb, oa_oracle_out *out) {
