@@ -162,7 +162,7 @@ DevScan oa::make_dev(const oagpu_ctx* c) {
   d.UA = c->UA; d.UA_invd = c->UA_invd; d.pat_pd = c->pat_pd;
   d.wtrait = c->wtrait; d.wpat = c->wpat; d.n_waves = c->n_waves;
   d.nU = c->compact_corr ? c->nU : 0;
-  d.uidx = c->uidx; d.XLu = c->XLu; d.pu_off = c->pu_off; d.pu_list = c->pu_list;
+  d.uidx = c->uidx; d.XLu = c->XLu; d.pu_off = c->pu_off; d.pu_list = c->pu_list; d.XLp = c->XLp;
   d.Xu = c->Xu; d.xu_dt = xu_dtype(c); d.ldxu = c->ldxu; d.pst = c->pst;
   d.Gm = c->Gm; d.gdt = c->gdt; d.ldg = c->ldg; d.col_src = c->col_src; d.col_env = c->col_env;
   d.g_vec = c->Gm && ((uintptr_t)c->Gm % 16 == 0) && (c->ldg % 4 == 0);
@@ -273,7 +273,7 @@ extern "C" void oagpu_destroy(oagpu_ctx* c) {
                   c->rss0, c->rt, c->zt, c->ay, c->hash, c->scratch_i, c->pat_cnt, c->pat_rep, c->pat_pd, c->keep,
                   c->UA, c->UA_invd, c->wtrait, c->wpat, c->Abuf, c->Cbuf,
                   c->col_mean, c->col_nan, c->Sbr, c->Str, c->varx, c->Ccorr, c->hits, c->counters, c->fb,
-                  c->uidx, c->pu_off, c->pu_list, c->XLu, c->Xu, c->pst, c->Bf};
+                  c->uidx, c->pu_off, c->pu_list, c->XLu, c->XLp, c->Xu, c->pst, c->Bf};
   for (void* b : bufs) dfree(b);
   if (c->st) (void)hipStreamDestroy(c->st);
   if (c->cst) (void)hipStreamDestroy(c->cst);
@@ -627,6 +627,8 @@ static int build_union(oagpu_ctx* c, const std::vector<int>& pat_rep, const std:
   HIPCK(c, hipMemcpyAsync(c->XLu, XLu.data(), sizeof(double) * XLu.size(), hipMemcpyHostToDevice, c->st));
   HIPCK(c, hipMemcpyAsync(c->uidx, uidx.data(), sizeof(int) * uidx.size(), hipMemcpyHostToDevice, c->st));
   launch_union_list(n, G, c->rows, c->pat_rep, c->pat_cnt, c->pu_off, c->uidx, c->pu_list, c->st);
+  HIPCK(c, grow((void**)&c->XLp, &c->XLp_cap, sizeof(double) * std::max<size_t>(tot * l, 1)));
+  launch_pattern_rows(c->pu_list, (long long)tot, l, c->XLu, c->XLp, c->st);
   HIPCK(c, hipGetLastError());
   HIPCK(c, hipStreamSynchronize(c->st));   // the host vectors above are released on return
   c->nU = nU;

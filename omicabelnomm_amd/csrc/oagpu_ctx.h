@@ -69,6 +69,8 @@ struct oagpu_ctx {
   int nU = 0;
   int *uidx = nullptr, *pu_off = nullptr, *pu_list = nullptr;
   double* XLu = nullptr;
+  double* XLp = nullptr;   // XLu rows of pu_list, pattern by pattern
+  size_t XLp_cap = 0;
   size_t uidx_cap = 0, pu_off_cap = 0, pu_list_cap = 0, XLu_cap = 0;
   int *pat_cnt = nullptr, *pat_rep = nullptr, *pat_pd = nullptr;
   double *UA = nullptr, *UA_invd = nullptr;

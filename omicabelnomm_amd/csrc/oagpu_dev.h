@@ -79,6 +79,7 @@ struct DevScan {
   const double* XLu;     // [nU][l] covariate rows of U
   const int* pu_off;     // [G+1] pattern g's rows are pu_list[pu_off[g]..pu_off[g+1])
   const int* pu_list;    // positions in U
+  const double* XLp;     // [pu_off[G]][l] covariate rows of pu_list's rows, in its order
 
   // genotypes
   const void* Gm; int gdt; long long ldg;

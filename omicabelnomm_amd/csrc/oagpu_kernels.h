@@ -22,6 +22,8 @@ void launch_union_mark(int n, int G, const int* rows, const int* pat_rep, const 
                        unsigned char* mark, hipStream_t st);
 void launch_union_list(int n, int G, const int* rows, const int* pat_rep, const int* pat_cnt, const int* off,
                        const int* uidx, int* pu_list, hipStream_t st);
+// XLp[i] = XLu[pu_list[i]] (l doubles per row) for the tot listed rows
+void launch_pattern_rows(const int* pu_list, long long tot, int l, const double* XLu, double* XLp, hipStream_t st);
 void launch_pattern(const DevScan& d, const double* Stl_pk, int G, const int* pat_rep, double* UA,
                     double* UA_invd, int* pat_pd, hipStream_t st);
 // zt = U_A^-T XL'y0, ay = U_A^-1 zt (covariate-only coefficients), rss0

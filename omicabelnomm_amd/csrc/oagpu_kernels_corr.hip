@@ -90,7 +90,9 @@ __global__ __launch_bounds__(256) void k_corr(DevScan d, int gq_count) {
   // genotype loads back to back instead of one dependent round trip per row.
   // The 64 rows' covariate rows are staged in LDS by the lanes (one row per
   // lane, zero past l and past the pattern's last row) and read back as
-  // wave-uniform broadcasts; the multiply-adds run without branches.
+  // wave-uniform broadcasts; the multiply-adds run without branches.  The
+  // rows come from XLp, the pattern's own covariate rows in list order (one
+  // load beside the row positions' load, not a gather that waits for them).
   constexpr int CORR_BATCH = 8;
   constexpr int STAGE = LB > 16 ? 32 : 64;   // rows staged at once (LDS: 4 waves x STAGE x LB doubles)
   __shared__ double xls[4][STAGE][LB];
@@ -102,7 +104,9 @@ __global__ __launch_bounds__(256) void k_corr(DevScan d, int gq_count) {
     const int my_u = lane < nb ? d.pu_list[i0 + lane] : d.nU;
     if (lane < STAGE) {
 #pragma unroll
-      for (int j = 0; j < LB; ++j) xw[lane][j] = (lane < nb && j < l) ? d.XLu[(size_t)my_u * l + j] : 0.0;
+      for (int j = 0; j < LB; ++j)
+        xw[lane][j] = (lane < nb && j < l) ? (d.XLp ? d.XLp[(size_t)(i0 + lane) * l + j] : d.XLu[(size_t)my_u * l + j])
+                                           : 0.0;
     }
     __builtin_amdgcn_wave_barrier();
     for (int b0 = 0; b0 < nb; b0 += CORR_BATCH) {

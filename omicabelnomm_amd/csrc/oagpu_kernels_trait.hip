@@ -7,6 +7,8 @@
 // The reference rebuilds the per-trait XL'XL correction for every trait;
 // here traits with the same missing rows share one pattern, and each
 // pattern's corrected covariate block Stl - C_ll is factored once.
+#include <algorithm>
+
 #include "oagpu_kernels.h"
 
 namespace oa {
@@ -243,6 +245,25 @@ void launch_union_mark(int n, int G, const int* rows, const int* pat_rep, const 
 void launch_union_list(int n, int G, const int* rows, const int* pat_rep, const int* pat_cnt, const int* off,
                        const int* uidx, int* pu_list, hipStream_t st) {
   hipLaunchKernelGGL(k_union_list, dim3(G), dim3(256), 0, st, n, rows, pat_rep, pat_cnt, off, uidx, pu_list);
+}
+
+// XLp[i][j] = XLu[pu_list[i]][j]: the covariate rows of every pattern's
+// missing rows in the patterns' own order, so the corrections read a
+// pattern's rows as one contiguous run instead of gathering them by position.
+__global__ __launch_bounds__(256) void k_pattern_rows(const int* pu_list, long long tot, int l, const double* XLu,
+                                                      double* XLp) {
+  for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < tot * l; q += (long long)gridDim.x * 256) {
+    const long long i = q / l;
+    const int j = (int)(q - i * l);
+    XLp[q] = XLu[(size_t)pu_list[i] * l + j];
+  }
+}
+
+void launch_pattern_rows(const int* pu_list, long long tot, int l, const double* XLu, double* XLp, hipStream_t st) {
+  if (tot <= 0) return;
+  const long long n = tot * l;
+  const int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_pattern_rows, dim3(blocks), dim3(256), 0, st, pu_list, tot, l, XLu, XLp);
 }
 
 void launch_pattern(const DevScan& d, const double* Stl_pk, int G, const int* pat_rep, double* UA,
