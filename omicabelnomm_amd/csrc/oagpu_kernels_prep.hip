@@ -393,8 +393,11 @@ __global__ __launch_bounds__(256, 2) void k_prep_pair(DevScan d) {
               *reinterpret_cast<float2*>(&xs[buf][uu[e] - umin][8 * w + 2 * q]) = make_float2((float)eg, (float)g);
         }
       if constexpr (CORR) if constexpr (STORE) {
+        // every range passes this barrier, with rows of U or without: the
+        // stores below read this buffer, and the range two later refills it
+        // only after the next range's barrier
+        __syncthreads();
         if (nr > 0) {   // block-uniform: every wave sees the same rows
-          __syncthreads();
           // 8 threads per row of U: columns 2 s_blk + 4 c .. + 3, those of
           // SNPs past the chunk left alone
           const int c = tid & 7, col = 2 * s_blk + 4 * c;
@@ -412,7 +415,8 @@ __global__ __launch_bounds__(256, 2) void k_prep_pair(DevScan d) {
     fetch(4 * lane, b0);
     fetch(4 * lane + 256, b1);
     // two buffers of the staged rows: a buffer is refilled two ranges later,
-    // after the next range's barrier, by which time every thread has stored it
+    // after the next range's barrier (taken by every range), by which time
+    // every thread has stored it
     for (int base = 0; base < d.n_pad; base += 512) {
       body(base, b0, 0);
       if (base + 256 < d.n_pad) body(base + 256, b1, CORR ? 1 : 0);
