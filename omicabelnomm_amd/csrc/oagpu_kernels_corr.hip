@@ -91,8 +91,9 @@ __global__ __launch_bounds__(256) void k_corr(DevScan d, int gq_count) {
   // The 64 rows' covariate rows are staged in LDS by the lanes (one row per
   // lane, zero past l and past the pattern's last row) and read back as
   // wave-uniform broadcasts; the multiply-adds run without branches.  The
-  // rows come from XLp, the pattern's own covariate rows in list order (one
-  // load beside the row positions' load, not a gather that waits for them).
+  // rows come from XLp, the pattern's own covariate rows in list order (read
+  // beside the row positions, not a gather that waits for them; the library
+  // builds XLp with the union, whenever these corrections run).
   constexpr int CORR_BATCH = 8;
   constexpr int STAGE = LB > 16 ? 32 : 64;   // rows staged at once (LDS: 4 waves x STAGE x LB doubles)
   __shared__ double xls[4][STAGE][LB];
@@ -100,13 +101,21 @@ __global__ __launch_bounds__(256) void k_corr(DevScan d, int gq_count) {
   const int beg = d.pu_off[g], end = d.pu_off[g + 1];
   for (int i0 = beg; i0 < end; i0 += STAGE) {
     const int nb = __builtin_amdgcn_readfirstlane(min(STAGE, end - i0));
-    // lanes past the pattern's end point at row nU of Xu, which is zero
-    const int my_u = lane < nb ? d.pu_list[i0 + lane] : d.nU;
+    const bool in = lane < nb;
+    // lanes past the pattern's end point at row nU of Xu, which is zero (the
+    // position is loaded in bounds and replaced, so no load waits in a branch)
+    const int pu = d.pu_list[i0 + (in ? lane : 0)];
+    const int my_u = in ? pu : d.nU;
     if (lane < STAGE) {
+      // every load of the lane's covariate row first, so they are in flight
+      // together, then the stores; in bounds: a lane past the end reads the
+      // stage's first row, an index past l the row's last value
+      const double* xr = d.XLp + (size_t)(i0 + (in ? lane : 0)) * l;
+      double v[LB];
 #pragma unroll
-      for (int j = 0; j < LB; ++j)
-        xw[lane][j] = (lane < nb && j < l) ? (d.XLp ? d.XLp[(size_t)(i0 + lane) * l + j] : d.XLu[(size_t)my_u * l + j])
-                                           : 0.0;
+      for (int j = 0; j < LB; ++j) v[j] = xr[min(j, l - 1)];
+#pragma unroll
+      for (int j = 0; j < LB; ++j) xw[lane][j] = (in && j < l) ? v[j] : 0.0;
     }
     __builtin_amdgcn_wave_barrier();
     for (int b0 = 0; b0 < nb; b0 += CORR_BATCH) {
